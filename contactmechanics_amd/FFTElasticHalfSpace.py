@@ -1,0 +1,346 @@
+"""FFT elastic half-spaces backed by the gfx950 HIP library.
+
+Drop-in for ``ContactMechanics/FFTElasticHalfSpace.py``:
+
+* ``PeriodicFFTElasticHalfSpace`` (reference :151-885; construction :169-306,
+  operators :573-611);
+* ``FreeFFTElasticHalfSpace`` (reference :888-1283; Hockney 2N padding
+  :950-951, real-space Love kernel :1033-1139, padded/unpadded
+  ``evaluate_disp`` :1141-1205, boundary check :1207-1283).
+
+The Green's function and every convolution live on the GPU (``cm_plan_create``
+/ ``cm_evaluate_disp`` / ``cm_evaluate_force``).  ``evaluate_disp`` and
+``evaluate_force`` accept NumPy arrays (returned as NumPy FP64, like the
+reference) or torch tensors on the HIP device (returned as device tensors in
+the plan precision, no host copies).
+
+Build-only keyword: ``precision="fp64"`` (default, every field and transform in
+FP64) or ``"fp32"`` (FP32 fields and transforms, FP64 reductions and scalars).
+"""
+
+import numpy as np
+
+from . import _native
+from .Substrates import ElasticSubstrate
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _tuple2(v, dim):
+    if not hasattr(v, "__iter__"):
+        v = (v,)
+    v = tuple(v)
+    return tuple(v[min(i, len(v) - 1)] for i in range(dim))
+
+
+class PeriodicFFTElasticHalfSpace(ElasticSubstrate):
+    """Periodic elastic half-space (Stanley & Kato).  See module docstring."""
+
+    name = "periodic_fft_elastic_halfspace"
+    _periodic = True
+    _bc = _native.CM_PERIODIC
+
+    def __init__(self, nb_grid_pts, young, physical_sizes=2 * np.pi, stiffness_q0=None,
+                 thickness=None, poisson=0.0, superclass=True, fft="serial", communicator=None,
+                 fftengine=None, precision="fp64", device=None):
+        super().__init__()
+        if not hasattr(nb_grid_pts, "__iter__"):
+            nb_grid_pts = (nb_grid_pts,)
+        self.__dim = len(nb_grid_pts)
+        if self.__dim != 2:
+            raise self.Error(
+                "Dimension of this problem is {}. Only 2-dimensional problems are supported "
+                "by the GPU solver".format(self.__dim))
+        if stiffness_q0 is not None and thickness is not None:
+            raise self.Error("Please specify either stiffness_q0 or thickness or neither.")
+        if communicator is not None and getattr(communicator, "size", 1) > 1:
+            raise self.Error("MPI domain decomposition is not supported; one GPU holds the whole grid")
+        self._nb_grid_pts = tuple(int(n) for n in nb_grid_pts)
+        self._physical_sizes = tuple(float(s) for s in _tuple2(physical_sizes, 2))
+        self._steps = tuple(s / n for s, n in zip(self._physical_sizes, self._nb_grid_pts))
+        self.young = young
+        self.poisson = poisson
+        self.contact_modulus = young / (1 - poisson ** 2)
+        self.stiffness_q0 = stiffness_q0
+        self.thickness = thickness
+        self._communicator = communicator
+        if precision not in ("fp64", "fp32"):
+            raise self.Error("precision must be 'fp64' or 'fp32'")
+        self.precision = precision
+        self._device = device
+        self._plan = None
+
+    # ---- grid properties (reference :308-430)
+    @property
+    def dim(self):
+        return self.__dim
+
+    @property
+    def nb_grid_pts(self):
+        return self._nb_grid_pts
+
+    @property
+    def physical_sizes(self):
+        return self._physical_sizes
+
+    @property
+    def area_per_pt(self):
+        return np.prod(self.physical_sizes) / np.prod(self.nb_grid_pts)
+
+    @property
+    def nb_domain_grid_pts(self):
+        return self.nb_grid_pts
+
+    @property
+    def nb_subdomain_grid_pts(self):
+        return self.nb_domain_grid_pts
+
+    @property
+    def topography_nb_subdomain_grid_pts(self):
+        return self.nb_subdomain_grid_pts
+
+    @property
+    def subdomain_locations(self):
+        return (0,) * self.dim
+
+    @property
+    def topography_subdomain_locations(self):
+        return self.subdomain_locations
+
+    @property
+    def subdomain_slices(self):
+        return tuple(slice(s, s + n) for s, n in zip(self.subdomain_locations, self.nb_subdomain_grid_pts))
+
+    @property
+    def topography_subdomain_slices(self):
+        return tuple(slice(s, s + n) for s, n in zip(self.topography_subdomain_locations,
+                                                      self.topography_nb_subdomain_grid_pts))
+
+    @property
+    def local_topography_subdomain_slices(self):
+        return tuple(slice(0, n) for n in self.topography_nb_subdomain_grid_pts)
+
+    @property
+    def nb_fourier_grid_pts(self):
+        mx, my = self.nb_domain_grid_pts
+        return (mx // 2 + 1, my)
+
+    @property
+    def fourier_locations(self):
+        return (0, 0)
+
+    @property
+    def fourier_slices(self):
+        return tuple(slice(0, n) for n in self.nb_fourier_grid_pts)
+
+    @property
+    def communicator(self):
+        return self._communicator
+
+    def __repr__(self):
+        size_str = ", ".join(f"{d}: {s}({n})" for d, s, n in zip("xy", self.physical_sizes, self.nb_grid_pts))
+        return f"{self.dim}-dimensional halfspace '{self.name}', physical_sizes(nb_grid_pts) in {size_str}, " \
+               f"E' = {self.young}"
+
+    # ---- device plan
+    def _plan_kwargs(self):
+        if self.stiffness_q0 is None:
+            q0_mode, q0 = 0, 0.0
+        elif self.stiffness_q0 == 0.0:
+            q0_mode, q0 = 1, 0.0
+        else:
+            q0_mode, q0 = 2, float(self.stiffness_q0)
+        return dict(poisson=self.poisson, q0_mode=q0_mode, stiffness_q0=q0, thickness=self.thickness)
+
+    @property
+    def plan(self):
+        """The device plan (built on first use: needs a HIP device)."""
+        if self._plan is None:
+            ctx = _native.Context.get(self._device)
+            prec = _native.CM_FP64 if self.precision == "fp64" else _native.CM_FP32
+            nx, ny = self.nb_grid_pts
+            sx, sy = self.physical_sizes
+            try:
+                self._plan = _native.Plan(ctx, nx, ny, self._bc, prec, sx, sy, self.young,
+                                          **self._plan_kwargs())
+            except _native.CmError as err:
+                raise self.Error(str(err)) from err
+        return self._plan
+
+    @property
+    def torch_dtype(self):
+        import torch
+        return torch.float64 if self.precision == "fp64" else torch.float32
+
+    def _device_str(self):
+        import torch
+        dev = self._device if self._device is not None else torch.cuda.current_device()
+        return f"cuda:{dev}"
+
+    def _kernel_ref_layout(self):
+        """Green's function (unscaled, real) in the reference's half-spectrum
+        layout ``(Mx//2+1, My)``: r2c along axis 0 (reference :119-128)."""
+        mx, my = self.nb_domain_grid_pts
+        g = self.plan.green_host()  # (my//2+1, mx), includes -1/(area*M)
+        g = g / (-1.0 / (self.area_per_pt * mx * my))
+        j = np.arange(my)
+        return g[np.minimum(j, my - j)][:, : mx // 2 + 1].T.copy()
+
+    @property
+    def greens_function(self):
+        return self._kernel_ref_layout()
+
+    @property
+    def surface_stiffness(self):
+        g = self._kernel_ref_layout()
+        s = np.zeros(g.shape, dtype=complex)
+        s[g != 0] = 1.0 / g[g != 0]
+        return s
+
+    # ---- operators
+    def _apply(self, arr, which):
+        import torch
+        expected = self.nb_subdomain_grid_pts
+        if tuple(arr.shape) != tuple(expected):
+            what = "force" if which == "disp" else "displacements"
+            raise self.Error(
+                f"{what} array has a different shape ({tuple(arr.shape)}) than this halfspace's "
+                f"nb_grid_pts ({expected})")
+        plan = self.plan
+        dev = self._device_str()
+        if _is_torch(arr):
+            src = arr.to(device=dev, dtype=self.torch_dtype).contiguous()
+            out = torch.empty_like(src)
+            ret_torch = True
+        else:
+            src = torch.from_numpy(np.ascontiguousarray(arr, dtype=np.float64)).to(
+                device=dev, dtype=self.torch_dtype)
+            out = torch.empty_like(src)
+            ret_torch = False
+        if which == "disp":
+            plan.evaluate_disp(src.data_ptr(), out.data_ptr())
+        else:
+            plan.evaluate_force(src.data_ptr(), out.data_ptr())
+        if ret_torch:
+            return out
+        return out.to(dtype=torch.float64).cpu().numpy()
+
+    def evaluate_disp(self, forces):
+        """Displacements due to pixel forces (reference :573-589)."""
+        return self._apply(forces, "disp")
+
+    def evaluate_force(self, disp):
+        """Pixel forces due to displacements (reference :591-611)."""
+        return self._apply(disp, "force")
+
+    def evaluate(self, disp, pot=True, forces=False):
+        """Elastic energy and/or forces of a displacement field.
+
+        Energy ``-sum(f u)/2`` with ``f = evaluate_force(u)`` in real space.
+        """
+        force = self.evaluate_force(disp)
+        energy = None
+        if pot:
+            energy = float(-0.5 * np.sum(np.asarray(force, dtype=float) * np.asarray(disp, dtype=float)))
+        return energy, (force if forces else None)
+
+
+class FreeFFTElasticHalfSpace(PeriodicFFTElasticHalfSpace):
+    """Non-periodic half-space via Hockney's 2N zero padding.  See module docstring."""
+
+    name = "free_fft_elastic_halfspace"
+    _periodic = False
+    _bc = _native.CM_FREE
+
+    class FreeBoundaryError(Exception):
+        """Forces overlap into the padding region (reference :1207-1215)."""
+
+    def __init__(self, nb_grid_pts, young, physical_sizes=2 * np.pi, fft="serial",
+                 communicator=None, check_boundaries=False, fftengine=None, precision="fp64",
+                 device=None):
+        super().__init__(nb_grid_pts, young, physical_sizes, fft=fft, communicator=communicator,
+                         precision=precision, device=device)
+        self._comp_nb_grid_pts = tuple(2 * r for r in self.nb_grid_pts)
+        self._check_boundaries = check_boundaries
+
+    def _plan_kwargs(self):
+        return dict(poisson=0.0, q0_mode=0, stiffness_q0=0.0, thickness=None)
+
+    def spawn_child(self, nb_grid_pts):
+        size = tuple(nb_grid_pts[i] / float(self.nb_grid_pts[i]) * self.physical_sizes[i]
+                     for i in range(self.dim))
+        return type(self)(nb_grid_pts, self.young, size, precision=self.precision, device=self._device)
+
+    @property
+    def nb_domain_grid_pts(self):
+        return self._comp_nb_grid_pts
+
+    @property
+    def topography_nb_subdomain_grid_pts(self):
+        return self.nb_grid_pts
+
+    @property
+    def domain_boundary_mask(self):
+        nx, ny = self.nb_grid_pts
+        mask = np.zeros((nx, ny), dtype=bool)
+        mask[:, 0] = True
+        if ny - 1 > 0:
+            mask[:, ny - 1] = True
+        mask[0, :] = True
+        if nx - 1 > 0:
+            mask[nx - 1, :] = True
+        return mask
+
+    def _kernel_ref_layout(self):
+        # the reference stores the complex FFT of the real even kernel; its
+        # imaginary part is round-off, the GPU keeps the real part
+        return super()._kernel_ref_layout()
+
+    def evaluate_disp(self, forces, bIncludePadding=False):
+        """Padded input -> padded output; unpadded input is zero-padded and the
+        output cropped unless ``bIncludePadding`` (reference :1141-1205)."""
+        shape = tuple(forces.shape)
+        if shape == self.nb_subdomain_grid_pts:
+            return super().evaluate_disp(forces)
+        if shape == self.nb_grid_pts:
+            nx, ny = self.nb_grid_pts
+            if _is_torch(forces):
+                import torch
+                padded = torch.zeros(self.nb_domain_grid_pts, dtype=self.torch_dtype,
+                                     device=self._device_str())
+                padded[:nx, :ny] = forces
+            else:
+                padded = np.zeros(self.nb_domain_grid_pts)
+                padded[:nx, :ny] = forces
+            u = super().evaluate_disp(padded)
+            return u if bIncludePadding else u[:nx, :ny]
+        raise self.Error(
+            "force array has a different shape ({0}) than the subdomain nb_grid_pts ({1}), this "
+            "halfspace's nb_grid_pts ({2}) or half of it.".format(
+                shape, self.nb_subdomain_grid_pts, self.nb_domain_grid_pts))
+
+    def check_boundaries(self, force=None, tol=0):
+        """Raise FreeBoundaryError if the force on the outer ring of the
+        topography exceeds ``tol`` (reference :1217-1269)."""
+        if force is None:
+            force = self.force
+        force = np.asarray(force)
+        nx, ny = self.nb_grid_pts
+        ok = bool((abs(force[:, 0]) <= tol).all())
+        if 0 < ny - 1 < force.shape[1]:
+            ok &= bool((abs(force[:, ny - 1]) <= tol).all())
+        ok &= bool((abs(force[0, :]) <= tol).all())
+        if 0 < nx - 1 < force.shape[0]:
+            ok &= bool((abs(force[nx - 1, :]) <= tol).all())
+        if not ok:
+            raise self.FreeBoundaryError(
+                "The forces not zero at the boundary of the active domain. This is typically an "
+                "indication that the contact geometry exceeds the bounds of the domain. Since this is "
+                "a nonperiodic calculation, you may want to increase the size of your domain. If you "
+                "are sure that the calculation is correct, set check_boundary to False")
+
+    def check(self, force=None):
+        if self._check_boundaries:
+            self.check_boundaries(force)

@@ -1,0 +1,229 @@
+"""Polonsky-Keer constrained conjugate gradients on the GPU.
+
+Drop-in for ``ContactMechanics/Optimization/ConstrainedConjugateGradients.py:47-509``:
+same signature, keyword defaults, ``OptimizeResult`` fields, messages and
+exceptions.  Every iteration runs on the device (``cm_ccg_solve``): the
+element-wise work is fused into the FFT passes and every global reduction and
+scalar decision (offset, step length, mixing switch, bisection, convergence)
+is computed in FP64 on the GPU.  The host only sets up buffers, waits, and
+converts the result.
+
+Extra keywords (not in the reference):
+
+``return_device``
+    keep ``x``/``jac``/``active_set`` as torch tensors on the device.
+``iters_per_sync``
+    host synchronisation interval; 0 = adaptive.  A ``callback`` or an active
+    ``logger`` forces 1 (the host then sees every iteration).
+``use_graph``
+    replay chunks of iterations from a captured hipGraph.
+"""
+
+import ctypes
+
+import numpy as np
+import scipy.optimize as optim
+
+from .. import _native
+from ..Tools.Logger import quiet
+
+
+def _heights_and_mask(topography):
+    if hasattr(topography, "nb_grid_pts"):
+        heights = topography.heights()
+    else:
+        heights = topography
+    mask = np.ma.getmask(heights)
+    if mask is np.ma.nomask:
+        return np.asarray(heights, dtype=np.float64), None, heights
+    valid = np.logical_not(mask)
+    return np.ascontiguousarray(np.ma.filled(heights, 0.0), dtype=np.float64), valid, heights
+
+
+def _rms_height(heights):
+    h = np.ma.compressed(np.ma.asarray(heights))
+    return float(np.sqrt(np.mean((h - h.mean()) ** 2)))
+
+
+def _to_device_field(arr, substrate, name):
+    import torch
+    dev = substrate._device_str()
+    shape = tuple(substrate.nb_subdomain_grid_pts)
+    if tuple(arr.shape) != shape:
+        raise substrate.Error(f"{name} has shape {tuple(arr.shape)}, expected {shape}")
+    if type(arr).__module__.startswith("torch"):
+        return arr.to(device=dev, dtype=substrate.torch_dtype).contiguous().clone()
+    return torch.from_numpy(np.ascontiguousarray(arr, dtype=np.float64)).to(
+        device=dev, dtype=substrate.torch_dtype)
+
+
+def constrained_conjugate_gradients(substrate, topography, hardness=None, external_force=None,
+                                    offset=None, initial_displacements=None, initial_forces=None,
+                                    pentol=None, forcetol=1e-5, thermotol=1e-6, mixfac=0.1,
+                                    mixdecfac=0.95, minmixsteps=10, maxiter=100000, logger=quiet,
+                                    callback=None, verbose=False, return_device=False,
+                                    iters_per_sync=0, use_graph=True):
+    """Equilibrium of a rigid ``topography`` pressed on an elastic ``substrate``.
+
+    See the reference docstring (``ConstrainedConjugateGradients.py:61-124``);
+    the returned ``OptimizeResult`` has ``x`` (displacements, full padded grid),
+    ``jac`` (forces on the topography, compressive positive), ``active_set``,
+    ``plastic`` (with hardness, on convergence), ``fun`` (elastic energy),
+    ``offset``, ``success``, ``message``, ``nit``, ``nfev``, ``maxcv``.
+    """
+    import torch
+
+    if substrate.nb_subdomain_grid_pts != substrate.nb_domain_grid_pts:  # :132-136
+        if not hasattr(topography, "nb_grid_pts"):
+            raise ValueError("You should provide a topography object when working with MPI")
+    if substrate.dim not in (1, 2):
+        raise Exception(f"Constrained conjugate gradient currently only implemented for 1 or 2 "
+                        f"dimensions (Your substrate has {substrate.dim}.).")
+    if not hasattr(substrate, "plan"):
+        raise TypeError("substrate must be a contactmechanics_amd FFT half-space")
+
+    h_np, valid, heights = _heights_and_mask(topography)
+    nx, ny = substrate.nb_grid_pts
+    if h_np.shape != (nx, ny):
+        raise substrate.Error(f"topography has shape {h_np.shape}, substrate expects {(nx, ny)}")
+    nb_surface_pts = nx * ny
+
+    if pentol is None:  # :150-162
+        pentol = _rms_height(heights) / (10 * np.mean(substrate.nb_grid_pts))
+        if pentol == 0:
+            pentol = ((offset if offset is not None else 0)
+                      + np.sum(np.ma.filled(heights, 0.0)) / nb_surface_pts) / 1000
+        if pentol == 0:
+            pentol = 1e-3
+    logger.pr(f"maxiter = {maxiter}")
+    logger.pr(f"pentol = {pentol}")
+
+    if hardness is not None and np.ndim(hardness) > 0:
+        harr = np.asarray(hardness, dtype=float)
+        if not np.all(harr == harr.flat[0]):
+            raise NotImplementedError("the GPU solver supports a scalar hardness")
+        hardness = float(harr.flat[0])
+
+    plan = substrate.plan
+    dev = substrate._device_str()
+    M_shape = tuple(substrate.nb_subdomain_grid_pts)
+    h_dev = torch.from_numpy(h_np).to(dev)
+    m_dev = None
+    if valid is not None:
+        m_dev = torch.from_numpy(np.ascontiguousarray(valid, dtype=np.uint8)).to(dev)
+    if initial_displacements is None:
+        u = torch.zeros(M_shape, dtype=substrate.torch_dtype, device=dev)
+    else:
+        u = _to_device_field(initial_displacements, substrate, "initial_displacements")
+    if initial_forces is None:
+        f = torch.zeros(M_shape, dtype=substrate.torch_dtype, device=dev)
+        init_mode = 0
+    else:
+        f = _to_device_field(initial_forces, substrate, "initial_forces")
+        init_mode = 1
+    cmask = torch.zeros(M_shape, dtype=torch.uint8, device=dev)
+
+    per_iteration = callback is not None or getattr(logger, "active", False)
+    params = _native.CcgParams(
+        load_control=1 if external_force is not None else 0,
+        external_force=float(external_force) if external_force is not None else 0.0,
+        offset=float(offset) if offset is not None else float("nan"),
+        has_hardness=1 if hardness is not None else 0,
+        hardness=float(hardness) if hardness is not None else 0.0,
+        pentol=float(pentol), forcetol=float(forcetol), thermotol=float(thermotol),
+        mixfac=float(mixfac), mixdecfac=float(mixdecfac), minmixsteps=int(minmixsteps),
+        maxiter=int(maxiter), init_mode=init_mode,
+        iters_per_sync=1 if per_iteration else int(iters_per_sync),
+        use_graph=0 if per_iteration else int(bool(use_graph)))
+    stats = _native.CcgStats()
+
+    n_surf = int(valid.sum()) if valid is not None else nb_surface_pts
+    errors = []
+
+    def progress(_user, st_ptr):
+        try:
+            st = st_ptr.contents
+            if st.status != _native.CM_RUNNING or st.nit == 0:
+                return  # finished: reported below
+            status = _native.DELTA_STR.get(st.last_mode, "?")
+            headers = ["status", "it", "area", "frac. area", "cg area", "total force", "offset"]
+            values = [status, st.nit, int(st.area), st.area / n_surf, int(st.cg_area), st.total_force,
+                      st.offset]
+            if hardness:
+                headers += ["plast. area", "frac.plast. area"]
+                values += [int(st.plastic_area), st.plastic_area / n_surf]
+            logger.st(headers, values)
+            if callback is not None:
+                d = dict(area=int(st.area), fractional_area=float(st.area / n_surf),
+                         rms_penetration=float(st.rms_pen), max_penetration=float(st.max_pen),
+                         max_pressure=float(st.max_pres), pad_pressure=float(st.pad_pres),
+                         penetration_tol=float(pentol), pressure_tol=float(forcetol))
+                callback(st.nit, f.to(torch.float64).cpu().numpy(), d)
+        except Exception as err:  # re-raised after the solve returns
+            errors.append(err)
+
+    cb = _native.PROGRESS_FN(progress) if per_iteration else _native.PROGRESS_FN()
+    # inputs were staged on torch's stream; the solve runs (and is captured
+    # into graphs) on the context's own non-blocking stream
+    stream = plan.ctx.run_begin()
+    rc = plan.lib.cm_ccg_solve(plan.handle, ctypes.byref(params), ctypes.c_void_p(h_dev.data_ptr()),
+                               ctypes.c_void_p(m_dev.data_ptr() if m_dev is not None else 0),
+                               ctypes.c_void_p(u.data_ptr()), ctypes.c_void_p(f.data_ptr()),
+                               ctypes.c_void_p(cmask.data_ptr()), ctypes.byref(stats), cb, None, stream)
+    plan.ctx.run_end()
+    if errors:
+        raise errors[0]
+    if rc == _native.CM_E_NAN:
+        raise RuntimeError("nan encountered.")
+    if rc == _native.CM_E_BISECT:
+        raise RuntimeError("bearing-area bisection failed: " + plan.lib.cm_last_error(plan.ctx.handle).decode())
+    if rc < 0:
+        _native.check(rc, plan.ctx.handle)
+
+    return _assemble_result(substrate, stats, u, f, cmask, hardness, return_device,
+                            logger, n_surf, maxiter)
+
+
+def _assemble_result(substrate, stats, u, f, cmask, hardness, return_device, logger, n_surf,
+                     maxiter):
+    import torch
+    nx, ny = substrate.nb_grid_pts
+    result = optim.OptimizeResult()
+    result.nit = int(stats.nit)
+    result.nfev = int(stats.nfev)
+    result.maxcv = {"max_pen": float(stats.max_pen), "max_pres": float(stats.max_pres)}
+    result.offset = float(stats.offset)
+    result.fun = float(stats.fun)
+    converged = stats.status == _native.CM_CONVERGED
+    mix_last = stats.last_mode == 3
+    active = (cmask & 1).bool()
+    gneg = ((cmask >> 1) & 1).bool()
+    if converged and mix_last:  # :468-469
+        active[:nx, :ny] = gneg[:nx, :ny]
+    jac = -f[:nx, :ny]
+    if return_device:
+        result.x, result.jac, result.active_set = u, jac, active
+        if converged and hardness is not None:
+            result.plastic = gneg[:nx, :ny].clone()
+    else:
+        result.x = u.to(torch.float64).cpu().numpy()
+        result.jac = jac.to(torch.float64).cpu().numpy()
+        result.active_set = active.cpu().numpy()
+        if converged and hardness is not None:  # :464-467
+            result.plastic = gneg[:nx, :ny].cpu().numpy()
+    headers = ["status", "it", "area", "frac. area", "cg area", "total force", "offset"]
+    values = ["CONVERGED" if converged else "NOT CONVERGED", result.nit, int(stats.area),
+              stats.area / max(n_surf, 1), int(stats.cg_area), stats.total_force, stats.offset]
+    logger.st(headers, values, force_print=True)
+    result.success = bool(converged)
+    if converged:
+        result.message = "Polonsky converged"
+    else:
+        result.message = "Reached maxiter = {}".format(maxiter)
+    return result
+
+
+# The reference cites these works through its @doi decorator.
+DOIS = ("10.1016/S0043-1648(99)00113-1",)
+PLASTIC_DOIS = ("10.1016/j.triboint.2005.11.008", "10.1038/s41467-018-02981-y")
+

@@ -1,0 +1,236 @@
+// Building blocks for the gfx950 contact solver: complex arithmetic, radix-R
+// DFT butterflies, an in-LDS Stockham autosort FFT engine and wavefront/block
+// reductions in FP64.
+//
+// The FFT engine transforms one "line" (a row or a column of the computational
+// grid) held in LDS by `tpl` cooperating threads.  Every thread owns E = L/tpl
+// elements; for a radix-R pass it runs E/R butterflies.  A pass loads all of a
+// thread's butterfly inputs into registers, synchronises, computes and writes
+// the outputs back in place (Stockham index map), then synchronises again, so
+// one LDS buffer of L complex values per line suffices.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cm {
+
+template <typename T> struct cplx { T x, y; };
+
+template <typename T> __device__ __forceinline__ cplx<T> mk(T a, T b) { cplx<T> r; r.x = a; r.y = b; return r; }
+template <typename T> __device__ __forceinline__ cplx<T> operator+(cplx<T> a, cplx<T> b) { return mk(a.x + b.x, a.y + b.y); }
+template <typename T> __device__ __forceinline__ cplx<T> operator-(cplx<T> a, cplx<T> b) { return mk(a.x - b.x, a.y - b.y); }
+template <typename T> __device__ __forceinline__ cplx<T> operator*(cplx<T> a, T s) { return mk(a.x * s, a.y * s); }
+template <typename T> __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+template <typename T> __device__ __forceinline__ cplx<T> cmulc(cplx<T> a, cplx<T> b) {
+  return mk(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <typename T> __device__ __forceinline__ cplx<T> conj(cplx<T> a) { return mk(a.x, -a.y); }
+// multiply by -i (forward) or +i (inverse)
+template <typename T, bool INV> __device__ __forceinline__ cplx<T> rot90(cplx<T> a) {
+  return INV ? mk(-a.y, a.x) : mk(a.y, -a.x);
+}
+
+// ---------------------------------------------------------------------------
+// Radix-R DFT of v[0..R-1] in registers.  Forward uses exp(-2 pi i/R),
+// inverse exp(+2 pi i/R).  Constants are the exact trigonometric values.
+template <typename T, int R, bool INV> struct Dft;
+
+template <typename T, bool INV> struct Dft<T, 2, INV> {
+  __device__ __forceinline__ static void run(cplx<T>* v) {
+    cplx<T> a = v[0], b = v[1];
+    v[0] = a + b; v[1] = a - b;
+  }
+};
+
+template <typename T, bool INV> struct Dft<T, 3, INV> {
+  __device__ __forceinline__ static void run(cplx<T>* v) {
+    const T c1 = T(-0.5);
+    const T s1 = T(0.86602540378443864676372317075294);  // sin(2pi/3)
+    cplx<T> a = v[0], b = v[1], c = v[2];
+    cplx<T> s = b + c, d = b - c;
+    cplx<T> t = a + s * c1;
+    cplx<T> e = rot90<T, INV>(d) * s1;  // -i*s1*(b-c) forward
+    v[0] = a + s;
+    v[1] = t + e;
+    v[2] = t - e;
+  }
+};
+
+template <typename T, bool INV> struct Dft<T, 4, INV> {
+  __device__ __forceinline__ static void run(cplx<T>* v) {
+    cplx<T> a = v[0], b = v[1], c = v[2], d = v[3];
+    cplx<T> s0 = a + c, s1 = a - c, s2 = b + d, s3 = rot90<T, INV>(b - d);
+    v[0] = s0 + s2; v[2] = s0 - s2;
+    v[1] = s1 + s3; v[3] = s1 - s3;
+  }
+};
+
+template <typename T, bool INV> struct Dft<T, 5, INV> {
+  __device__ __forceinline__ static void run(cplx<T>* v) {
+    const T c1 = T(0.30901699437494742410229341718282);   // cos(2pi/5)
+    const T c2 = T(-0.80901699437494742410229341718282);  // cos(4pi/5)
+    const T s1 = T(0.95105651629515357211643933337938);   // sin(2pi/5)
+    const T s2 = T(0.58778525229247312916870595463907);   // sin(4pi/5)
+    cplx<T> a = v[0];
+    cplx<T> p1 = v[1] + v[4], m1 = v[1] - v[4];
+    cplx<T> p2 = v[2] + v[3], m2 = v[2] - v[3];
+    cplx<T> t1 = a + p1 * c1 + p2 * c2;
+    cplx<T> t2 = a + p1 * c2 + p2 * c1;
+    // forward: X1 = t1 - i(s1 m1 + s2 m2), X4 = t1 + i(...)
+    //          X2 = t2 - i(s2 m1 - s1 m2), X3 = t2 + i(...)
+    cplx<T> e1 = rot90<T, INV>(m1 * s1 + m2 * s2);
+    cplx<T> e2 = rot90<T, INV>(m1 * s2 - m2 * s1);
+    v[0] = a + p1 + p2;
+    v[1] = t1 + e1; v[4] = t1 - e1;
+    v[2] = t2 + e2; v[3] = t2 - e2;
+  }
+};
+
+template <typename T, bool INV> struct Dft<T, 8, INV> {
+  __device__ __forceinline__ static void run(cplx<T>* v) {
+    const T h = T(0.70710678118654752440084436210485);
+    // radix-2 across stride 4, then twiddle, then two radix-4
+    cplx<T> a0 = v[0] + v[4], a4 = v[0] - v[4];
+    cplx<T> a1 = v[1] + v[5], a5 = v[1] - v[5];
+    cplx<T> a2 = v[2] + v[6], a6 = v[2] - v[6];
+    cplx<T> a3 = v[3] + v[7], a7 = v[3] - v[7];
+    // twiddles w8^k for the odd half: k=1: (h, -h) fwd, k=2: -i, k=3: (-h,-h) fwd
+    a5 = INV ? mk(h * (a5.x - a5.y), h * (a5.x + a5.y)) : mk(h * (a5.x + a5.y), h * (a5.y - a5.x));
+    a6 = rot90<T, INV>(a6);
+    a7 = INV ? mk(-h * (a7.x + a7.y), h * (a7.x - a7.y)) : mk(h * (a7.y - a7.x), -h * (a7.x + a7.y));
+    cplx<T> e[4] = {a0, a1, a2, a3};
+    cplx<T> o[4] = {a4, a5, a6, a7};
+    Dft<T, 4, INV>::run(e);
+    Dft<T, 4, INV>::run(o);
+    v[0] = e[0]; v[2] = e[1]; v[4] = e[2]; v[6] = e[3];
+    v[1] = o[0]; v[3] = o[1]; v[5] = o[2]; v[7] = o[3];
+  }
+};
+
+// ---------------------------------------------------------------------------
+// FFT plan for one line length, passed by value to kernels.
+struct LinePlan {
+  int L;          // transform length
+  int tpl;        // threads per line
+  int E;          // elements per thread (L / tpl)
+  int npass;      // number of radix passes
+  int radix[14];  // radix of each pass (product == L)
+};
+
+// Twiddle table: tw[k] = exp(-2 pi i k / L), k in [0, L).
+// One Stockham pass (radix R, sub-transform length Ns before the pass).
+// Line element n lives at lds[base + n*stride].
+template <typename T, int R, bool INV, int EMAX>
+__device__ __forceinline__ void stockham_pass(cplx<T>* lds, int base, int stride, int L, int Ns,
+                                              int tl, int tpl, int nb, const cplx<T>* __restrict__ tw) {
+  constexpr int NBMAX = EMAX / R > 0 ? EMAX / R : 1;
+  cplx<T> v[NBMAX][R];
+  const int LR = L / R;
+#pragma unroll
+  for (int b = 0; b < NBMAX; ++b) {
+    if (b < nb) {
+      const int j = tl + b * tpl;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b][r] = lds[base + (j + r * LR) * stride];
+    }
+  }
+  __syncthreads();
+  const int twstep = L / (Ns * R);
+#pragma unroll
+  for (int b = 0; b < NBMAX; ++b) {
+    if (b < nb) {
+      const int j = tl + b * tpl;
+      const int k = j % Ns;
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          cplx<T> w = tw[r * k * twstep];
+          v[b][r] = INV ? cmulc(v[b][r], w) : cmul(v[b][r], w);
+        }
+      }
+      Dft<T, R, INV>::run(v[b]);
+      const int o = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[base + (o + r * Ns) * stride] = v[b][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Full in-place LDS FFT of one line (all threads of the block must call it;
+// threads whose tl >= tpl must pass active=false and still hit the barriers).
+template <typename T, bool INV, int EMAX>
+__device__ void fft_line(cplx<T>* lds, int base, int stride, const LinePlan& p, int tl, bool active,
+                         const cplx<T>* __restrict__ tw) {
+  int Ns = 1;
+  for (int s = 0; s < p.npass; ++s) {
+    const int R = p.radix[s];
+    const int nb = active ? p.E / R : 0;
+    switch (R) {
+      case 2: stockham_pass<T, 2, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
+      case 3: stockham_pass<T, 3, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
+      case 4: stockham_pass<T, 4, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
+      case 5: stockham_pass<T, 5, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
+      default: stockham_pass<T, 8, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
+    }
+    Ns *= R;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Reductions in FP64.  A reduction record holds NS slots; OPS[i] is the op.
+enum RedOp : int { RSUM = 0, RMAX = 1, RMIN = 2 };
+
+__device__ __forceinline__ double red_identity(int op) {
+  return op == RSUM ? 0.0 : (op == RMAX ? -__builtin_huge_val() : __builtin_huge_val());
+}
+__device__ __forceinline__ double red_apply(int op, double a, double b) {
+  // NaN-propagating max/min so that a NaN anywhere reaches the scalar logic
+  if (op == RSUM) return a + b;
+  if (a != a || b != b) return a + b;
+  return op == RMAX ? (a > b ? a : b) : (a < b ? a : b);
+}
+
+__device__ __forceinline__ double wave_reduce(int op, double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = red_apply(op, v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Block-wide reduction of NS slots; result valid in thread 0.
+// `scratch` must hold NS * (blockDim/64) doubles of shared memory.
+template <int NS>
+__device__ __forceinline__ void block_reduce(double* vals, const int* ops, double* scratch) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    double v = wave_reduce(ops[i], vals[i]);
+    if (lane == 0) scratch[i * 16 + wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      double v = scratch[i * 16];
+      for (int w = 1; w < nw; ++w) v = red_apply(ops[i], v, scratch[i * 16 + w]);
+      vals[i] = v;
+    }
+  }
+  __syncthreads();
+}
+
+// Neumaier-compensated accumulation for per-thread sums.
+struct Kahan {
+  double s = 0.0, c = 0.0;
+  __device__ __forceinline__ void add(double x) {
+    double t = s + x;
+    if (fabs(s) >= fabs(x)) c += (s - t) + x; else c += (x - t) + s;
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return s + c; }
+};
+
+}  // namespace cm

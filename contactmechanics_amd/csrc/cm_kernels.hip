@@ -1,0 +1,1647 @@
+// gfx950 kernels and C ABI of the contact solver (libcmhip.so).
+//
+// Convolution u = G (*) f on an (Mx, My) real grid, My even, runs as three
+// passes over HBM:
+//   1. row pass  (k_r2c): real rows -> half-length complex FFT in LDS ->
+//      real-to-complex split -> spectrum S in a column-blocked layout;
+//   2. column pass (k_col): forward FFT along x, multiply by the real Green's
+//      function G(kx, ky), inverse FFT, all inside LDS;
+//   3. row pass  (k_c2r): complex-to-real merge -> inverse FFT -> real rows.
+// Row passes take a "producer"/"consumer" functor so the CCG element-wise work
+// and its FP64 reductions are fused into the loads/stores of the transforms.
+//
+// Spectrum layout: S holds Mx x (My/2) complex values.  Column ky = 0 packs the
+// real DC (ky = 0) and Nyquist (ky = My/2) columns as one complex column.
+// Element (x, ky) lives at ((ky / Cb) * Mx + x) * Cb + ky % Cb: each column
+// block of Cb columns is contiguous, so the column pass reads and writes one
+// contiguous slab and the row pass writes R*Cb-element contiguous runs.
+//
+// Reference: ContactMechanics/FFTElasticHalfSpace.py (operators) and
+// ContactMechanics/Optimization/ConstrainedConjugateGradients.py (solver);
+// each CCG phase below cites the lines it implements.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/cmhip.h"
+#include "cm_fft.hpp"
+
+namespace cm {
+
+// ---------------------------------------------------------------------------
+// Geometry and per-instance state
+
+struct Geom {
+  int nx, ny;   // topography grid
+  int Mx, My;   // computational grid
+  int N2;       // My / 2 (complex row length)
+  int Cb;       // spectrum column-block width
+  int R;        // rows per row-pass block
+};
+
+enum { DS_RESET = 0, DS_SD = 1, DS_CG = 2, DS_MIX = 3 };
+enum { MODE_PK = 0, MODE_MIX = 1 };
+
+// Scalar state of one CCG solve (device resident).  Doubles first.
+struct Scal {
+  // parameters
+  double F, H, pentol, forcetol, thermotol, mixfac, mixdecfac;
+  // constants from set-up
+  double max_h, nb_surface_pts, n_pad;
+  // iteration state
+  double offset, shift, G, G_old, beta, tau, mix, last_tf, last_mh;
+  // per-iteration values
+  double A_contact, A_cg, A_fl, Scgt, gmax, gmin, max_pres, pad_pres, scale, total_force;
+  double nc_count, maxdu, max_pen, rms_pen, fun;
+  int load_ctl, has_H, H_truthy, minmixsteps, maxiter;
+  int it, nfev, delta, delta_str, mode, mixsteps, done, status;
+  int has_last_tf, has_last_mh, use_beta, do_nc, uniform, pad_;
+};
+
+template <typename T>
+struct Inst {
+  T* u;            // displacements (M grid)
+  T* f;            // forces (M grid)
+  T* t;            // conjugate direction (M grid)
+  uint8_t* cm;     // bit0 contact set c, bit1 gap<0 at start of iteration
+  const double* h; // heights (nx*ny)
+  const uint8_t* surf;  // valid-pixel mask (nx*ny) or null
+  cplx<T>* S;      // spectrum work buffer
+  const T* G;      // evaluate_disp kernel, (N2+1) x Mx
+  const T* K;      // evaluate_force kernel, (N2+1) x Mx
+  const T* src;    // operator input (plain evaluate)
+  T* dst;          // operator output (plain evaluate)
+  Scal* sc;
+  double* part;    // per-block partial reductions
+  unsigned* ctr;   // last-block tickets (8 slots)
+};
+
+template <typename T>
+struct InstArr {
+  const Inst<T>* arr;  // device array (batched) or null
+  Inst<T> one;         // single instance by value
+  __device__ __forceinline__ const Inst<T>& get(int b) const { return arr ? arr[b] : one; }
+};
+
+__device__ __forceinline__ bool is_comp(const Geom& g, const uint8_t* surf, int i, int j) {
+  return i < g.nx && j < g.ny && (surf == nullptr || surf[i * g.ny + j] != 0);
+}
+
+// ---------------------------------------------------------------------------
+// Cross-block reduction: every block reduces NS slots, writes its partials,
+// takes a ticket; the block drawing the last ticket reduces all partials in a
+// fixed order (deterministic result) and returns true.
+template <class P>
+__device__ bool reduce_blocks(double* acc, double* part, unsigned* ctr, int nblocks, int bid) {
+  constexpr int NS = P::NS;
+  __shared__ double scr[8 * 16];
+  __shared__ int s_last;
+  int ops[NS > 0 ? NS : 1];
+#pragma unroll
+  for (int i = 0; i < NS; ++i) ops[i] = P::op(i);
+  block_reduce<NS>(acc, ops, scr);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NS; ++i) part[(size_t)bid * NS + i] = acc[i];
+    __threadfence();
+    unsigned tk = atomicAdd(ctr, 1u);
+    s_last = (tk == (unsigned)(nblocks - 1));
+  }
+  __syncthreads();
+  if (!s_last) return false;
+  __threadfence();
+  double v[NS > 0 ? NS : 1];
+#pragma unroll
+  for (int i = 0; i < NS; ++i) v[i] = red_identity(ops[i]);
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+      v[i] = red_apply(ops[i], v[i],
+                       __hip_atomic_load(&part[(size_t)b * NS + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  }
+  block_reduce<NS>(v, ops, scr);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NS; ++i) acc[i] = v[i];
+    __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Row pass: real rows -> packed half spectrum (forward, unnormalised).
+template <typename T, int EMAX, class P>
+__global__ void __launch_bounds__(256) k_r2c(Geom g, LinePlan lp, const cplx<T>* __restrict__ tw,
+                                              const cplx<T>* __restrict__ twp, InstArr<T> ia, int ctr_slot) {
+  extern __shared__ __align__(16) unsigned char smem_r2c[];
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_r2c);
+  const Inst<T>& in = ia.get(blockIdx.y);
+  P p(in);
+  if (p.skip()) return;
+  const int N2 = g.N2;
+  const int r = threadIdx.x / lp.tpl, tl = threadIdx.x - r * lp.tpl;
+  const int row = blockIdx.x * g.R + r;
+  double acc[P::NS > 0 ? P::NS : 1];
+#pragma unroll
+  for (int i = 0; i < P::NS; ++i) acc[i] = red_identity(P::op(i));
+  cplx<T>* L = lds + r * N2;
+  for (int n = tl; n < N2; n += lp.tpl) {
+    T a, b;
+    p.load(g, row, 2 * n, a, b, acc);
+    L[n] = mk(a, b);
+  }
+  __syncthreads();
+  fft_line<T, false, EMAX>(lds, r * N2, 1, lp, tl, true, tw);
+  // split the half-length transform into the real-input spectrum X[k],
+  // k = 0..N2, packing X[0] (real) and X[N2] (real) into slot 0.
+  for (int k = tl; k <= N2 / 2; k += lp.tpl) {
+    if (k == 0) {
+      cplx<T> z = L[0];
+      L[0] = mk(z.x + z.y, z.x - z.y);
+    } else {
+      const int km = N2 - k;
+      cplx<T> zk = L[k], zm = L[km];
+      cplx<T> E = (zk + conj(zm)) * T(0.5);
+      cplx<T> D = zk - conj(zm);
+      cplx<T> O = mk(D.y * T(0.5), -D.x * T(0.5));
+      cplx<T> B = cmul(twp[k], O);
+      L[k] = E + B;
+      if (km != k) L[km] = conj(E - B);
+    }
+  }
+  __syncthreads();
+  cplx<T>* S = in.S;
+  const int RC = g.R * g.Cb;
+  const int tot = g.R * N2;
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int kk = i % g.Cb, rr = (i / g.Cb) % g.R, cb = i / RC;
+    S[((size_t)cb * g.Mx + blockIdx.x * g.R + rr) * g.Cb + kk] = lds[rr * N2 + cb * g.Cb + kk];
+  }
+  if constexpr (P::NS > 0) {
+    if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
+      p.finalize(acc);
+  }
+}
+
+// Column pass.  MODE 0: forward * G * inverse; 1: forward only; 2: inverse only.
+// skip: 0 never, 1 when the solve is done, 2 when done or in a mixing step.
+template <typename T, int EMAX, int MODE>
+__global__ void __launch_bounds__(256) k_col(Geom g, LinePlan lp, const cplx<T>* __restrict__ tw,
+                                              InstArr<T> ia, int useK, int skip) {
+  extern __shared__ __align__(16) unsigned char smem_col[];
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_col);
+  const Inst<T>& in = ia.get(blockIdx.y);
+  if (skip && in.sc) {
+    if (in.sc->done) return;
+    if (skip == 2 && in.sc->mode == MODE_MIX) return;
+  }
+  cplx<T>* S = in.S;
+  const T* G = useK ? in.K : in.G;
+  const int cb = blockIdx.x;
+  const int Mx = g.Mx, Cb = g.Cb;
+  const size_t base = (size_t)cb * Mx * Cb;
+  const int n = Mx * Cb;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = S[base + i];
+  __syncthreads();
+  const int c = threadIdx.x % Cb, tl = threadIdx.x / Cb;
+  if (MODE != 2) fft_line<T, false, EMAX>(lds, c, Cb, lp, tl, true, tw);
+  if (MODE == 0) {
+    const int ky = cb * Cb + c;
+    if (ky != 0) {
+      const T* Gk = G + (size_t)ky * Mx;
+      for (int x = tl; x < Mx; x += lp.tpl) lds[x * Cb + c] = lds[x * Cb + c] * Gk[x];
+    } else {
+      // packed DC/Nyquist column: Q = A + iB with A, B spectra of real
+      // columns; A *= G(kx, 0), B *= G(kx, My/2).
+      const T* G0 = G;
+      const T* GN = G + (size_t)g.N2 * Mx;
+      for (int x = tl; x <= Mx / 2; x += lp.tpl) {
+        const int xm = (Mx - x) % Mx;
+        cplx<T> q = lds[x * Cb], qm = lds[xm * Cb];
+        const T a0 = G0[x], aN = GN[x], b0 = G0[xm], bN = GN[xm];
+        cplx<T> nq = q * (T(0.5) * (a0 + aN)) + conj(qm) * (T(0.5) * (a0 - aN));
+        cplx<T> nqm = qm * (T(0.5) * (b0 + bN)) + conj(q) * (T(0.5) * (b0 - bN));
+        lds[x * Cb] = nq;
+        if (xm != x) lds[xm * Cb] = nqm;
+      }
+    }
+    __syncthreads();
+  }
+  if (MODE != 1) fft_line<T, true, EMAX>(lds, c, Cb, lp, tl, true, tw);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) S[base + i] = lds[i];
+}
+
+// Row pass: packed half spectrum -> real rows (inverse, unnormalised; the
+// round trip r2c -> c2r scales by Mx*My, which the kernels G/K absorb).
+template <typename T, int EMAX, class P>
+__global__ void __launch_bounds__(256) k_c2r(Geom g, LinePlan lp, const cplx<T>* __restrict__ tw,
+                                              const cplx<T>* __restrict__ twp, InstArr<T> ia, int ctr_slot) {
+  extern __shared__ __align__(16) unsigned char smem_c2r[];
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_c2r);
+  const Inst<T>& in = ia.get(blockIdx.y);
+  P p(in);
+  if (p.skip()) return;
+  const int N2 = g.N2;
+  const cplx<T>* S = in.S;
+  const int RC = g.R * g.Cb;
+  const int tot = g.R * N2;
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int kk = i % g.Cb, rr = (i / g.Cb) % g.R, cb = i / RC;
+    lds[rr * N2 + cb * g.Cb + kk] = S[((size_t)cb * g.Mx + blockIdx.x * g.R + rr) * g.Cb + kk];
+  }
+  __syncthreads();
+  const int r = threadIdx.x / lp.tpl, tl = threadIdx.x - r * lp.tpl;
+  const int row = blockIdx.x * g.R + r;
+  cplx<T>* L = lds + r * N2;
+  for (int k = tl; k <= N2 / 2; k += lp.tpl) {
+    if (k == 0) {
+      cplx<T> z = L[0];  // (X[0], X[N2])
+      L[0] = mk(z.x + z.y, z.x - z.y);
+    } else {
+      const int km = N2 - k;
+      cplx<T> xk = L[k], xm = L[km];
+      cplx<T> E = xk + conj(xm);
+      cplx<T> O = cmulc(xk - conj(xm), twp[k]);
+      L[k] = mk(E.x - O.y, E.y + O.x);            // E + iO
+      if (km != k) L[km] = mk(E.x + O.y, O.x - E.y);  // conj(E) + i conj(O)
+    }
+  }
+  __syncthreads();
+  fft_line<T, true, EMAX>(lds, r * N2, 1, lp, tl, true, tw);
+  double acc[P::NS > 0 ? P::NS : 1];
+#pragma unroll
+  for (int i = 0; i < P::NS; ++i) acc[i] = red_identity(P::op(i));
+  for (int n = tl; n < N2; n += lp.tpl) {
+    cplx<T> z = L[n];
+    p.store(g, row, 2 * n, z.x, z.y, acc);
+  }
+  if constexpr (P::NS > 0) {
+    if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
+      p.finalize(acc);
+  }
+}
+
+// Grid-stride element-wise kernel with fused reductions.
+template <typename T, class P>
+__global__ void __launch_bounds__(256) k_elem(Geom g, InstArr<T> ia, int ctr_slot) {
+  const Inst<T>& in = ia.get(blockIdx.y);
+  P p(in);
+  if (p.skip()) return;
+  double acc[P::NS > 0 ? P::NS : 1];
+#pragma unroll
+  for (int i = 0; i < P::NS; ++i) acc[i] = red_identity(P::op(i));
+  const size_t M = (size_t)g.Mx * g.My;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < M; idx += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / g.My), j = (int)(idx - (size_t)i * g.My);
+    p.point(g, idx, i, j, acc);
+  }
+  if constexpr (P::NS > 0) {
+    if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
+      p.finalize(acc);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Functors for plain operator evaluation
+
+template <typename T>
+struct ProdCopy {  // input row from src
+  static constexpr int NS = 0;
+  __device__ static constexpr int op(int) { return RSUM; }
+  const Inst<T>& in;
+  __device__ ProdCopy(const Inst<T>& i) : in(i) {}
+  __device__ bool skip() const { return false; }
+  __device__ void load(const Geom& g, int row, int col, T& a, T& b, double*) const {
+    const size_t p = (size_t)row * g.My + col;
+    a = in.src[p];
+    b = in.src[p + 1];
+  }
+  __device__ void finalize(double*) const {}
+};
+
+template <typename T>
+struct ConsCopy {  // output row to dst
+  static constexpr int NS = 0;
+  __device__ static constexpr int op(int) { return RSUM; }
+  const Inst<T>& in;
+  __device__ ConsCopy(const Inst<T>& i) : in(i) {}
+  __device__ bool skip() const { return false; }
+  __device__ void store(const Geom& g, int row, int col, T a, T b, double*) const {
+    const size_t p = (size_t)row * g.My + col;
+    in.dst[p] = a;
+    in.dst[p + 1] = b;
+  }
+  __device__ void finalize(double*) const {}
+};
+
+// ---------------------------------------------------------------------------
+// CCG set-up (ConstrainedConjugateGradients.py:179-219)
+
+// u[comp] = max(u, h + offset) (:201); max over valid heights (:196-197).
+template <typename T>
+struct ElemInit {
+  static constexpr int NS = 2;  // max_h, n_comp
+  __device__ static constexpr int op(int i) { return i == 0 ? RMAX : RSUM; }
+  const Inst<T>& in;
+  __device__ ElemInit(const Inst<T>& i) : in(i) {}
+  __device__ bool skip() const { return false; }
+  __device__ void point(const Geom& g, size_t p, int i, int j, double* acc) const {
+    if (!is_comp(g, in.surf, i, j)) return;
+    const double hv = in.h[i * g.ny + j];
+    const double lim = hv + in.sc->offset;
+    const double uv = (double)in.u[p];
+    if (uv < lim) in.u[p] = (T)lim;
+    acc[0] = red_apply(RMAX, acc[0], hv);
+    acc[1] += 1.0;
+  }
+  // the host stores the full grid size M in n_pad; subtract the comp count
+  __device__ void finalize(double* acc) const {
+    Scal* s = in.sc;
+    s->max_h = acc[0];
+    s->n_pad = s->n_pad - acc[1];
+  }
+};
+
+// f = evaluate_force(u) output with f[pad] = 0 (:212, :219).
+template <typename T>
+struct ConsInitForce {
+  static constexpr int NS = 0;
+  __device__ static constexpr int op(int) { return RSUM; }
+  const Inst<T>& in;
+  __device__ ConsInitForce(const Inst<T>& i) : in(i) {}
+  __device__ bool skip() const { return false; }
+  __device__ void store(const Geom& g, int row, int col, T a, T b, double*) const {
+    const size_t p = (size_t)row * g.My + col;
+    in.f[p] = is_comp(g, in.surf, row, col) ? a : T(0);
+    in.f[p + 1] = is_comp(g, in.surf, row, col + 1) ? b : T(0);
+  }
+  __device__ void finalize(double*) const {}
+};
+
+// f[pad] = 0 after u = evaluate_disp(initial_forces) (:213-219).
+template <typename T>
+struct ElemZeroPad {
+  static constexpr int NS = 0;
+  __device__ static constexpr int op(int) { return RSUM; }
+  const Inst<T>& in;
+  __device__ ElemZeroPad(const Inst<T>& i) : in(i) {}
+  __device__ bool skip() const { return false; }
+  __device__ void point(const Geom& g, size_t p, int i, int j, double*) const {
+    if (!is_comp(g, in.surf, i, j)) in.f[p] = T(0);
+  }
+  __device__ void finalize(double*) const {}
+};
+
+// ---------------------------------------------------------------------------
+// CCG iteration phases
+
+// Phase A: contact set and gap statistics (:238-273, :312-318 brackets).
+// c = f<0 [& f>-H]; A_contact, A_cg; shifted gap sums for offset and G.
+template <typename T>
+struct ElemPhaseA {
+  static constexpr int NS = 6;  // A_contact, A_cg, Sd, Sdd, gmax, gmin
+  __device__ static constexpr int op(int i) { return i == 4 ? RMAX : (i == 5 ? RMIN : RSUM); }
+  const Inst<T>& in;
+  __device__ ElemPhaseA(const Inst<T>& i) : in(i) {}
+  __device__ bool skip() const { return in.sc->done != 0; }
+  __device__ void point(const Geom& g, size_t p, int i, int j, double* acc) const {
+    const Scal* s = in.sc;
+    const double fv = (double)in.f[p];
+    const bool c0 = fv < 0.0;
+    const bool c = c0 && (!s->has_H || fv > -s->H);
+    acc[0] += c0 ? 1.0 : 0.0;
+    acc[1] += c ? 1.0 : 0.0;
+    in.cm[p] = c ? 1 : 0;
+    if (is_comp(g, in.surf, i, j)) {
+      const double g0 = (double)in.u[p] - in.h[i * g.ny + j];
+      if (c) {
+        const double d = g0 - s->shift;
+        acc[2] += d;
+        acc[3] += d * d;
+      }
+      acc[4] = red_apply(RMAX, acc[4], g0);
+      acc[5] = red_apply(RMIN, acc[5], g0);
+    }
+  }
+  __device__ void finalize(double* a) const {
+    Scal* s = in.sc;
+    s->it += 1;  // iteration number of this pass (:235-236)
+    s->A_contact = a[0];
+    s->A_cg = a[1];
+    s->gmax = a[4];
+    s->gmin = a[5];
+    const bool pk = !s->has_H || (a[0] > 0 && a[1] / a[0] > 0.01 && s->mixsteps <= 0);  // :260
+    if (pk) {
+      s->mode = MODE_PK;
+      if (s->delta_str == DS_MIX) s->delta_str = DS_SD;  // :261-262
+      if (s->load_ctl) s->offset = a[1] > 0 ? s->shift + a[2] / a[1] : 0.0;  // :266-269
+      const double m = s->offset - s->shift;
+      double G = a[3] - 2.0 * m * a[2] + a[1] * m * m;  // sum_c (g - offset)^2 (:273)
+      s->G = G > 0.0 ? G : 0.0;
+      s->use_beta = (s->delta > 0 && s->G_old > 0) ? 1 : 0;  // :276
+      s->beta = s->use_beta ? s->delta * (s->G / s->G_old) : 0.0;
+    } else {
+      s->mode = MODE_MIX;
+      if (s->delta_str != DS_MIX) {  // :307-309
+        s->mixsteps = s->minmixsteps;
+        s->delta_str = DS_MIX;
+      }
+    }
+  }
+};
+
+// scipy.optimize.bisect (xtol=2e-12, rtol=4*eps, maxiter=100) of
+// x -> count(g - x < 0) * H - F over the computational region (:316-318).
+// One block per instance.
+template <typename T>
+__global__ void __launch_bounds__(1024) k_bisect(Geom g, InstArr<T> ia) {
+  const Inst<T>& in = ia.get(blockIdx.x);
+  Scal* s = in.sc;
+  if (s->done || s->mode != MODE_MIX || !s->load_ctl) return;
+  __shared__ double scr[8 * 16];
+  __shared__ double s_cnt;
+  const size_t Mt = (size_t)g.nx * g.ny;
+  const double H = s->H, F = s->F;
+  auto fval = [&](double x) -> double {
+    double v = 0.0;
+    for (size_t q = threadIdx.x; q < Mt; q += blockDim.x) {
+      const int i = (int)(q / g.ny), j = (int)(q - (size_t)i * g.ny);
+      if (in.surf && !in.surf[q]) continue;
+      const double g0 = (double)in.u[(size_t)i * g.My + j] - in.h[q];
+      v += (g0 - x < 0.0) ? 1.0 : 0.0;
+    }
+    int ops[1] = {RSUM};
+    block_reduce<1>(&v, ops, scr);
+    if (threadIdx.x == 0) s_cnt = v;
+    __syncthreads();
+    const double cnt = s_cnt;
+    __syncthreads();
+    return cnt * H - F;
+  };
+  double xa = s->gmax, xb = s->gmin;
+  const double fa = fval(xa), fb = fval(xb);
+  int status = 0;
+  double res = xa;
+  if (fa * fb > 0) {
+    status = CM_E_BISECT;
+  } else if (fa == 0) {
+    res = xa;
+  } else if (fb == 0) {
+    res = xb;
+  } else {
+    double dm = xb - xa;
+    bool ok = false;
+    for (int i = 0; i < 100; ++i) {
+      dm *= 0.5;
+      const double xm = xa + dm;
+      const double fm = fval(xm);
+      if (fm * fa >= 0) xa = xm;
+      if (fm == 0 || fabs(dm) < 2e-12 + 8.881784197001252e-16 * fabs(xm)) {
+        res = xm;
+        ok = true;
+        break;
+      }
+    }
+    if (!ok) status = CM_E_BISECT;
+  }
+  if (threadIdx.x == 0) {
+    if (status) {
+      s->status = status;
+      s->done = 1;
+    } else {
+      s->offset = res;
+    }
+  }
+}
+
+// Conjugate direction t = c (g + beta t) on comp (:265-279) fused into the
+// first forward transform; partial sum_c g t for tau (:292).
+template <typename T>
+struct ProdDir {
+  static constexpr int NS = 1;
+  __device__ static constexpr int op(int) { return RSUM; }
+  const Inst<T>& in;
+  __device__ ProdDir(const Inst<T>& i) : in(i) {}
+  __device__ bool skip() const { return in.sc->done || in.sc->mode == MODE_MIX; }
+  __device__ T one(const Geom& g, int row, int col, double* acc) const {
+    const size_t p = (size_t)row * g.My + col;
+    if (!is_comp(g, in.surf, row, col)) return in.t[p];
+    const Scal* s = in.sc;
+    const bool c = (in.cm[p] & 1) != 0;
+    const double gap = ((double)in.u[p] - in.h[row * g.ny + col]) - s->offset;
+    double tn = 0.0;
+    if (c) tn = s->use_beta ? gap + s->beta * (double)in.t[p] : gap;
+    const T tt = (T)tn;
+    in.t[p] = tt;
+    if (c) acc[0] += gap * (double)tt;
+    return tt;
+  }
+  __device__ void load(const Geom& g, int row, int col, T& a, T& b, double* acc) const {
+    a = one(g, row, col, acc);
+    b = one(g, row, col + 1, acc);
+  }
+  __device__ void finalize(double* a) const { in.sc->Scgt = a[0]; }
+};
+
+// r = evaluate_disp(t) consumed on the fly: sum c r t and the step length tau
+// (:284-294).
+template <typename T>
+struct ConsTau {
+  static constexpr int NS = 1;
+  __device__ static constexpr int op(int) { return RSUM; }
+  const Inst<T>& in;
+  __device__ ConsTau(const Inst<T>& i) : in(i) {}
+  __device__ bool skip() const { return in.sc->done || in.sc->mode == MODE_MIX; }
+  __device__ void store(const Geom& g, int row, int col, T a, T b, double* acc) const {
+    const size_t p = (size_t)row * g.My + col;
+    if (in.cm[p] & 1) acc[0] += (double)a * (double)in.t[p];
+    if (in.cm[p + 1] & 1) acc[0] += (double)b * (double)in.t[p + 1];
+  }
+  __device__ void finalize(double* a) const {
+    Scal* s = in.sc;
+    s->nfev += 1;
+    s->tau = 0.0;
+    if (s->A_cg > 0) {
+      const double x = -a[0];
+      if (x > 0.0)
+        s->tau = s->Scgt / x;
+      else
+        s->G = 0.0;
+    }
+  }
+};
+
+// Per-pixel force after the PK or mixing update, before projection
+// (:296 and :322-323).  Shared by phase B and the second transform.
+template <typename T>
+struct ForceUpdate {
+  __device__ static __forceinline__ double f1(const Scal* s, double fv, double tv, bool c) {
+    if (s->mode == MODE_PK) return fv + (c ? s->tau * tv : 0.0);
+    return (1.0 - s->mix) * fv - (s->mix * s->H) * (c ? 1.0 : 0.0);
+  }
+};
+
+// Phase B: projection statistics (:332-389).
+template <typename T>
+struct ElemPhaseB {
+  static constexpr int NS = 7;  // n_tensile, max_tensile, pad_max, A_fl, min_fl, sum_f2, nc
+  __device__ static constexpr int op(int i) {
+    return (i == 1 || i == 2) ? RMAX : (i == 4 ? RMIN : RSUM);
+  }
+  const Inst<T>& in;
+  __device__ ElemPhaseB(const Inst<T>& i) : in(i) {}
+  __device__ bool skip() const { return in.sc->done != 0; }
+  __device__ void point(const Geom& g, size_t p, int i, int j, double* acc) const {
+    const Scal* s = in.sc;
+    const bool comp = is_comp(g, in.surf, i, j);
+    double gap = 0.0;
+    if (comp) gap = ((double)in.u[p] - in.h[i * g.ny + j]) - s->offset;
+    bool c = (in.cm[p] & 1) != 0;
+    if (s->mode == MODE_MIX && comp) {  // c[comp] = g < 0 (:322)
+      c = gap < 0.0;
+      in.cm[p] = c ? 1 : 0;
+    }
+    const double f1 = ForceUpdate<T>::f1(s, (double)in.f[p], (double)in.t[p], c);
+    const bool tensile = f1 >= 0.0;
+    const bool flowing = s->has_H && f1 <= -s->H;
+    if (tensile) {
+      acc[0] += 1.0;
+      acc[1] = red_apply(RMAX, acc[1], f1);
+    }
+    if (!comp) acc[2] = red_apply(RMAX, acc[2], fabs(f1));
+    if (s->H_truthy && flowing) {
+      acc[3] += 1.0;
+      acc[4] = red_apply(RMIN, acc[4], f1 + s->H);
+    }
+    if (comp) {
+      acc[5] += tensile ? 0.0 : f1;
+      const bool nc = (tensile && gap < 0.0) || (flowing && gap > 0.0);
+      acc[6] += nc ? 1.0 : 0.0;
+    }
+  }
+  __device__ void finalize(double* a) const {
+    Scal* s = in.sc;
+    s->pad_pres = s->n_pad > 0 ? a[2] : 0.0;             // :344-346
+    s->max_pres = a[0] > 0 ? a[1] : 0.0;                 // :350-352
+    s->A_fl = a[3];
+    if (s->H_truthy && a[3] > 0) {                       // :353-356
+      const double v = -a[4];
+      s->max_pres = (v > s->max_pres) ? v : s->max_pres;
+    }
+    s->uniform = 0;
+    s->scale = 1.0;
+    if (s->load_ctl) {                                   // :362-368
+      const double tf = -a[5];
+      if (tf != 0)
+        s->scale = s->F / tf;
+      else
+        s->uniform = 1;
+    }
+    s->nc_count = a[6];
+    s->do_nc = 0;
+    if (s->mode == MODE_MIX) {                           // :375-377
+      s->delta = 0;
+      s->G = 0;
+    } else if (a[6] > 0) {                               // :379-386
+      s->do_nc = 1;
+      s->delta = 0;
+      s->delta_str = DS_SD;
+    } else {                                             // :387-389
+      s->delta = 1;
+      s->delta_str = DS_CG;
+    }
+  }
+};
+
+// Final projected force (:359-384) fused into the second forward transform.
+template <typename T>
+struct ProdForce {
+  static constexpr int NS = 1;  // sum f over comp
+  __device__ static constexpr int op(int) { return RSUM; }
+  const Inst<T>& in;
+  __device__ ProdForce(const Inst<T>& i) : in(i) {}
+  __device__ bool skip() const { return in.sc->done != 0; }
+  __device__ T one(const Geom& g, int row, int col, double* acc) const {
+    const Scal* s = in.sc;
+    const size_t p = (size_t)row * g.My + col;
+    const bool comp = is_comp(g, in.surf, row, col);
+    double gap = 0.0;
+    if (comp) gap = ((double)in.u[p] - in.h[row * g.ny + col]) - s->offset;
+    const bool c = (in.cm[p] & 1) != 0;
+    const double f1 = ForceUpdate<T>::f1(s, (double)in.f[p], (double)in.t[p], c);
+    const bool tensile = f1 >= 0.0;
+    const bool flowing = s->has_H && f1 <= -s->H;
+    double fv = tensile ? 0.0 : f1;                       // :359
+    if (s->load_ctl) {
+      if (s->uniform)
+        fv = comp ? -s->F / s->nb_surface_pts : 0.0;      // :367-368
+      else
+        fv = fv * s->scale;                               // :365
+    }
+    if (flowing) fv = -s->H;                              // :372-373
+    if (comp) {
+      const bool nc = (tensile && gap < 0.0) || (flowing && gap > 0.0);
+      if (s->do_nc && nc) fv = fv + s->tau * gap;         // :384
+      in.cm[p] = (uint8_t)((c ? 1 : 0) | (gap < 0.0 ? 2 : 0));
+    }
+    const T ft = (T)fv;
+    in.f[p] = ft;
+    if (comp) acc[0] += (double)ft;
+    return ft;
+  }
+  __device__ void load(const Geom& g, int row, int col, T& a, T& b, double* acc) const {
+    a = one(g, row, col, acc);
+    b = one(g, row, col + 1, acc);
+  }
+  __device__ void finalize(double* a) const { in.sc->total_force = -a[0]; }
+};
+
+// u_new = evaluate_disp(f) consumed on the fly: maxdu, max_pen and the
+// convergence tests (:391-434), NaN check (:491-492), maxiter (:494-509).
+template <typename T>
+struct ConsDisp {
+  static constexpr int NS = 2;  // maxdu, max_pen
+  __device__ static constexpr int op(int) { return RMAX; }
+  const Inst<T>& in;
+  __device__ ConsDisp(const Inst<T>& i) : in(i) {}
+  __device__ bool skip() const { return in.sc->done != 0; }
+  __device__ void one(const Geom& g, int row, int col, T v, double* acc) const {
+    const size_t p = (size_t)row * g.My + col;
+    const double un = (double)v;
+    acc[0] = red_apply(RMAX, acc[0], fabs(un - (double)in.u[p]));
+    in.u[p] = v;
+    if (is_comp(g, in.surf, row, col)) {
+      const bool c = (in.cm[p] & 1) != 0;
+      const double pen = c ? (in.h[row * g.ny + col] + in.sc->offset - un) : 0.0;
+      acc[1] = red_apply(RMAX, acc[1], pen);
+    }
+  }
+  __device__ void store(const Geom& g, int row, int col, T a, T b, double* acc) const {
+    one(g, row, col, a, acc);
+    one(g, row, col + 1, b, acc);
+  }
+  __device__ void finalize(double* a) const {
+    Scal* s = in.sc;
+    s->nfev += 1;
+    s->maxdu = a[0];
+    s->G_old = s->G;
+    s->rms_pen = s->A_cg > 0 ? sqrt(s->G / s->A_cg) : sqrt(s->G);
+    s->max_pen = a[1] > 0.0 ? a[1] : 0.0;
+    bool conv = true;
+    const double tf = s->total_force;
+    if (s->load_ctl)
+      conv = conv && fabs((tf - s->F) / tf) < s->thermotol;
+    else if (s->has_last_tf)
+      conv = conv && fabs((tf - s->last_tf) / tf) < s->thermotol;
+    s->last_tf = tf;
+    s->has_last_tf = 1;
+    const double mh = s->max_h + s->offset;
+    if (s->load_ctl && s->has_last_mh) conv = conv && fabs((mh - s->last_mh) / mh) < s->thermotol;
+    s->last_mh = mh;
+    s->has_last_mh = 1;
+    if (s->mode == MODE_MIX)
+      conv = conv && s->maxdu < s->pentol && s->max_pres < s->forcetol && s->pad_pres < s->forcetol;
+    else
+      conv = conv && s->rms_pen < s->pentol && s->max_pen < s->pentol && s->maxdu < s->pentol &&
+             s->max_pres < s->forcetol && s->pad_pres < s->forcetol;
+    if (s->mode == MODE_MIX) {  // :326-330
+      s->mix *= s->mixdecfac;
+      if (s->mixsteps > 0) s->mixsteps -= 1;
+    } else {
+      s->mix = s->mixfac;  // :299
+    }
+    s->shift = s->offset;
+    if (conv) {
+      s->status = CM_CONVERGED;
+      s->done = 1;
+    } else if (s->G != s->G || s->rms_pen != s->rms_pen) {
+      s->status = CM_E_NAN;
+      s->done = 1;
+    } else if (s->it >= s->maxiter) {
+      s->status = CM_MAXITER;
+      s->done = 1;
+    }
+  }
+};
+
+// Elastic energy -sum(f u)/2 over the topography block (:471, :505-506).
+template <typename T>
+struct ElemEnergy {
+  static constexpr int NS = 1;
+  __device__ static constexpr int op(int) { return RSUM; }
+  const Inst<T>& in;
+  __device__ ElemEnergy(const Inst<T>& i) : in(i) {}
+  __device__ bool skip() const { return false; }
+  __device__ void point(const Geom& g, size_t p, int i, int j, double* acc) const {
+    if (i < g.nx && j < g.ny) acc[0] += (double)in.f[p] * (double)in.u[p];
+  }
+  __device__ void finalize(double* a) const { in.sc->fun = -a[0] / 2; }
+};
+
+// ---------------------------------------------------------------------------
+// Green's function set-up kernels
+
+// Love's displacement under unit pressure on one pixel, on the wrapped 2N
+// grid (FFTElasticHalfSpace.py:1041-1137).
+__global__ void k_love_kernel(int nx, int ny, double dx, double dy, double young, double* out) {
+  const int Mx = 2 * nx, My = 2 * ny;
+  const size_t M = (size_t)Mx * My;
+  const double a = dx * 0.5, b = dy * 0.5;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < M; idx += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / My), j = (int)(idx - (size_t)i * My);
+    const double x = (i <= nx ? i : i - Mx) * dx;
+    const double y = (j <= ny ? j : j - My) * dy;
+    const double xp = x + a, xm = x - a, yp = y + b, ym = y - b;
+    auto term = [](double p, double q, double r, double s) {
+      return p * log((q + sqrt(q * q + p * p)) / (r + sqrt(r * r + s * s)));
+    };
+    out[idx] = (term(xp, yp, ym, xp) + term(yp, xp, xm, yp) + term(xm, ym, yp, xm) + term(ym, xm, xp, ym)) /
+               (M_PI * young);
+  }
+}
+
+// Real part of the packed forward spectrum in the (N2+1) x Mx Green layout.
+__global__ void k_unpack_green(Geom g, const cplx<double>* S, double* out) {
+  const size_t n = (size_t)(g.N2 + 1) * g.Mx;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    const int ky = (int)(idx / g.Mx), x = (int)(idx - (size_t)ky * g.Mx);
+    double v;
+    if (ky > 0 && ky < g.N2) {
+      v = S[((size_t)(ky / g.Cb) * g.Mx + x) * g.Cb + ky % g.Cb].x;
+    } else {
+      const int xm = (g.Mx - x) % g.Mx;
+      cplx<double> q = S[(size_t)x * g.Cb], qm = S[(size_t)xm * g.Cb];
+      v = ky == 0 ? 0.5 * (q.x + qm.x) : 0.5 * (q.y + qm.y);
+    }
+    out[idx] = v;
+  }
+}
+
+}  // namespace cm
+
+// ===========================================================================
+// Host side
+
+using namespace cm;
+
+struct cm_ctx {
+  int device = 0;
+  size_t budget = 0;
+  size_t used = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::mutex mu;
+};
+
+namespace {
+
+thread_local std::string g_err_noctx;
+
+int fail(cm_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->err = msg;
+  } else {
+    g_err_noctx = msg;
+  }
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                       \
+  do {                                                                                          \
+    hipError_t e_ = (expr);                                                                     \
+    if (e_ != hipSuccess)                                                                       \
+      return fail((ctx), CM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+int dev_alloc(cm_ctx* ctx, void** p, size_t bytes) {
+  if (ctx->budget && ctx->used + bytes > ctx->budget)
+    return fail(ctx, CM_E_OOM, "device memory budget exceeded");
+  hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+  if (e != hipSuccess) return fail(ctx, CM_E_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  ctx->used += bytes;
+  return 0;
+}
+void dev_free(cm_ctx* ctx, void* p, size_t bytes) {
+  if (p) {
+    (void)hipFree(p);
+    ctx->used -= std::min(ctx->used, bytes);
+  }
+}
+
+// Choose radices and threads-per-line for a length L: every radix must divide
+// the elements-per-thread E so each thread runs whole butterflies.  Fewer
+// passes first, then E close to `e_pref`.
+bool make_line_plan(int L, int max_tpl, int e_pref, LinePlan& best) {
+  static const int Es[] = {16, 32, 12, 24, 20, 8, 10, 6, 4, 2};
+  bool found = false;
+  int best_score = 1 << 30;
+  for (int E : Es) {
+    if (L % E != 0) continue;
+    const int tpl = L / E;
+    if (tpl > max_tpl) continue;
+    LinePlan lp{};
+    lp.L = L;
+    lp.E = E;
+    lp.tpl = tpl;
+    int rem = L, np = 0;
+    bool ok = true;
+    while (rem > 1) {
+      int pick = 0;
+      for (int R : {8, 4, 5, 3, 2})
+        if (E % R == 0 && rem % R == 0) {
+          pick = R;
+          break;
+        }
+      if (!pick || np >= 14) {
+        ok = false;
+        break;
+      }
+      lp.radix[np++] = pick;
+      rem /= pick;
+    }
+    if (!ok) continue;
+    lp.npass = np;
+    const int score = np * 100 + std::abs(E - e_pref);
+    if (score < best_score) {
+      best_score = score;
+      best = lp;
+      found = true;
+    }
+  }
+  return found;
+}
+
+template <typename T>
+struct TwTables {
+  cplx<T>* row = nullptr;   // N2 entries
+  cplx<T>* col = nullptr;   // Mx entries
+  cplx<T>* post = nullptr;  // N2/2+1 entries: exp(-2 pi i k / My)
+};
+
+template <typename T>
+std::vector<cplx<T>> host_twiddles(int L, int n, int denom) {
+  std::vector<cplx<T>> v(n);
+  const long double twopi = 6.283185307179586476925286766559005768L;
+  for (int k = 0; k < n; ++k) {
+    const long double ang = twopi * (long double)k / (long double)denom;
+    v[k].x = (T)cosl(ang);
+    v[k].y = (T)(-sinl(ang));
+  }
+  (void)L;
+  return v;
+}
+
+}  // namespace
+
+// One FFT executor per precision.
+struct FftExec {
+  Geom g{};
+  LinePlan row{}, col{};
+  int emax_row = 16, emax_col = 16;
+  void* tw_row = nullptr;
+  void* tw_col = nullptr;
+  void* tw_post = nullptr;
+  size_t lds_row = 0, lds_col = 0;
+};
+
+struct cm_plan {
+  cm_ctx* ctx = nullptr;
+  cm_substrate_desc d{};
+  int Mx = 0, My = 0;
+  double area_per_pt = 0;
+  int prec = CM_FP64;
+  FftExec ex;            // executor in plan precision
+  void* G = nullptr;     // (N2+1) x Mx, plan precision, scaled for evaluate_disp
+  void* K = nullptr;     // (N2+1) x Mx, plan precision, scaled for evaluate_force
+  std::vector<double> G_host;  // FP64 copy (scaled) for inspection
+  void* S = nullptr;     // spectrum work buffer for plain operator calls
+  size_t bytes_owned = 0;
+  int nblk_elem = 1;
+  // persistent solve workspace (grown on demand)
+  size_t ws_B = 0;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+};
+
+namespace {
+
+template <typename T>
+int exec_setup(cm_ctx* ctx, FftExec& ex, int nx, int ny, int Mx, int My, size_t& owned) {
+  Geom& g = ex.g;
+  g.nx = nx;
+  g.ny = ny;
+  g.Mx = Mx;
+  g.My = My;
+  g.N2 = My / 2;
+  const size_t cs = sizeof(cplx<T>);
+  constexpr int kMaxBlock = 256;  // FFT kernels are built with __launch_bounds__(256)
+  if (!make_line_plan(g.N2, kMaxBlock, 16, ex.row) || !make_line_plan(Mx, kMaxBlock, cs == 8 ? 32 : 16, ex.col))
+    return fail(ctx, CM_E_UNSUPPORTED,
+                "grid " + std::to_string(Mx) + "x" + std::to_string(My) +
+                    " not supported: Mx and My/2 must factor into 2, 3, 5");
+  // column block width: contiguous slab of Mx*Cb complex values in LDS
+  int Cb = 1;
+  const int cb_target = (int)(64 / cs) > 0 ? (int)(64 / cs) : 1;  // 64-byte row segments
+  while (Cb * 2 <= cb_target && g.N2 % (Cb * 2) == 0 && (size_t)Mx * Cb * 2 * cs <= 65536 &&
+         ex.col.tpl * Cb * 2 <= kMaxBlock)
+    Cb *= 2;
+  if ((size_t)Mx * Cb * cs > 160 * 1024) return fail(ctx, CM_E_UNSUPPORTED, "column length exceeds LDS capacity");
+  g.Cb = Cb;
+  int R = 1;
+  while (Mx % (R * 2) == 0 && ex.row.tpl * R * 2 <= kMaxBlock && (size_t)R * 2 * g.N2 * cs <= 65536 &&
+         (ex.row.tpl * R < 128 || (size_t)R * Cb * cs < 128))
+    R *= 2;
+  g.R = R;
+  ex.lds_row = (size_t)R * g.N2 * cs;
+  ex.lds_col = (size_t)Mx * Cb * cs;
+  ex.emax_row = ex.row.E <= 16 ? 16 : 32;
+  ex.emax_col = ex.col.E <= 16 ? 16 : 32;
+  auto twr = host_twiddles<T>(g.N2, g.N2, g.N2);
+  auto twc = host_twiddles<T>(Mx, Mx, Mx);
+  auto twp = host_twiddles<T>(My, g.N2 / 2 + 1, My);
+  int rc;
+  if ((rc = dev_alloc(ctx, &ex.tw_row, twr.size() * cs))) return rc;
+  if ((rc = dev_alloc(ctx, &ex.tw_col, twc.size() * cs))) return rc;
+  if ((rc = dev_alloc(ctx, &ex.tw_post, twp.size() * cs))) return rc;
+  owned += (twr.size() + twc.size() + twp.size()) * cs;
+  HIPCHK(ctx, hipMemcpy(ex.tw_row, twr.data(), twr.size() * cs, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(ex.tw_col, twc.data(), twc.size() * cs, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(ex.tw_post, twp.data(), twp.size() * cs, hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Allow more than 64 KB of dynamic LDS for a kernel (once per kernel).
+void allow_big_lds(const void* kern, size_t lds) {
+  if (lds <= 65536) return;
+  static std::mutex mu;
+  static std::vector<const void*> done;
+  std::lock_guard<std::mutex> lk(mu);
+  if (std::find(done.begin(), done.end(), kern) != done.end()) return;
+  (void)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  done.push_back(kern);
+}
+
+template <typename T, class P>
+void launch_r2c(const FftExec& ex, InstArr<T> ia, int B, int slot, hipStream_t st) {
+  dim3 grid(ex.g.Mx / ex.g.R, B), block(ex.row.tpl * ex.g.R);
+  const cplx<T>* tw = (const cplx<T>*)ex.tw_row;
+  const cplx<T>* twp = (const cplx<T>*)ex.tw_post;
+  if (ex.emax_row == 16) {
+    allow_big_lds((const void*)k_r2c<T, 16, P>, ex.lds_row);
+    hipLaunchKernelGGL((k_r2c<T, 16, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
+  } else {
+    allow_big_lds((const void*)k_r2c<T, 32, P>, ex.lds_row);
+    hipLaunchKernelGGL((k_r2c<T, 32, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
+  }
+}
+
+template <typename T, class P>
+void launch_c2r(const FftExec& ex, InstArr<T> ia, int B, int slot, hipStream_t st) {
+  dim3 grid(ex.g.Mx / ex.g.R, B), block(ex.row.tpl * ex.g.R);
+  const cplx<T>* tw = (const cplx<T>*)ex.tw_row;
+  const cplx<T>* twp = (const cplx<T>*)ex.tw_post;
+  if (ex.emax_row == 16) {
+    allow_big_lds((const void*)k_c2r<T, 16, P>, ex.lds_row);
+    hipLaunchKernelGGL((k_c2r<T, 16, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
+  } else {
+    allow_big_lds((const void*)k_c2r<T, 32, P>, ex.lds_row);
+    hipLaunchKernelGGL((k_c2r<T, 32, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
+  }
+}
+
+template <typename T, int MODE>
+void launch_col(const FftExec& ex, InstArr<T> ia, int B, int useK, int skip, hipStream_t st) {
+  dim3 grid(ex.g.N2 / ex.g.Cb, B), block(ex.col.tpl * ex.g.Cb);
+  const cplx<T>* tw = (const cplx<T>*)ex.tw_col;
+  if (ex.emax_col == 16) {
+    allow_big_lds((const void*)k_col<T, 16, MODE>, ex.lds_col);
+    hipLaunchKernelGGL((k_col<T, 16, MODE>), grid, block, ex.lds_col, st, ex.g, ex.col, tw, ia, useK, skip);
+  } else {
+    allow_big_lds((const void*)k_col<T, 32, MODE>, ex.lds_col);
+    hipLaunchKernelGGL((k_col<T, 32, MODE>), grid, block, ex.lds_col, st, ex.g, ex.col, tw, ia, useK, skip);
+  }
+}
+
+template <typename T, class P>
+void launch_elem(const FftExec& ex, int nblk, InstArr<T> ia, int B, int slot, hipStream_t st) {
+  dim3 grid(nblk, B), block(256);
+  hipLaunchKernelGGL((k_elem<T, P>), grid, block, 0, st, ex.g, ia, slot);
+}
+
+int plan_free(cm_plan* p) {
+  if (!p) return 0;
+  cm_ctx* ctx = p->ctx;
+  dev_free(ctx, p->ex.tw_row, 0);
+  dev_free(ctx, p->ex.tw_col, 0);
+  dev_free(ctx, p->ex.tw_post, 0);
+  dev_free(ctx, p->G, 0);
+  dev_free(ctx, p->K, 0);
+  dev_free(ctx, p->S, 0);
+  dev_free(ctx, p->ws, 0);
+  ctx->used -= std::min(ctx->used, p->bytes_owned + p->ws_bytes);
+  delete p;
+  return 0;
+}
+
+// Periodic Green's function on the host in FP64 (FFTElasticHalfSpace.py:494-561)
+// in the (N2+1) x Mx layout: ky along the r2c axis, x along the full axis.
+void periodic_green_host(const cm_substrate_desc& d, int Mx, int My, std::vector<double>& G) {
+  const int N2 = My / 2;
+  G.assign((size_t)(N2 + 1) * Mx, 0.0);
+  const double Es = d.young / (1 - d.poisson * d.poisson);
+  const bool thick = d.thickness > 0 && std::isfinite(d.thickness);
+  auto qcomp = [](int k, int n, double s) { return (k <= n / 2 ? k : n - k) / s; };
+  for (int ky = 0; ky <= N2; ++ky) {
+    const double qy = qcomp(ky, My, d.sy);
+    for (int x = 0; x < Mx; ++x) {
+      const double qx = qcomp(x, Mx, d.sx);
+      double q = std::sqrt(qx * qx + qy * qy);
+      double stiff;
+      if (x == 0 && ky == 0) {
+        if (thick) {
+          stiff = d.young / d.thickness * (1 - d.poisson) / ((1 - 2 * d.poisson) * (1 + d.poisson));
+        } else if (d.q0_mode == 0) {
+          stiff = (M_PI * Es * (1.0 / d.sx) + M_PI * Es * (1.0 / d.sy)) / 2;
+        } else if (d.q0_mode == 1) {
+          stiff = 1.0;
+        } else {
+          stiff = d.stiffness_q0;
+        }
+      } else {
+        stiff = M_PI * Es * q;
+        if (thick) {
+          const double qt = q * 2 * M_PI * d.thickness;
+          const double fac = 3 - 4 * d.poisson;
+          const double off = 4 * d.poisson * (2 * d.poisson - 3) + 5;
+          double corr = (fac * std::cosh(2 * qt) + 2 * qt * qt + off) / (fac * std::sinh(2 * qt) - 2 * qt);
+          if (std::isnan(corr)) corr = 1.0;
+          stiff *= corr;
+        }
+      }
+      double gv = 1.0 / stiff;
+      if (x == 0 && ky == 0 && !thick && d.q0_mode == 1) gv = 0.0;
+      G[(size_t)ky * Mx + x] = gv;
+    }
+  }
+}
+
+// Free Green's function: Love kernel on the 2N grid, FP64 forward transform
+// on the device, real part unpacked (FFTElasticHalfSpace.py:1033-1139).
+int free_green_device(cm_plan* P, std::vector<double>& G) {
+  cm_ctx* ctx = P->ctx;
+  FftExec ex;
+  size_t owned = 0;
+  int rc = exec_setup<double>(ctx, ex, P->d.nx, P->d.ny, P->Mx, P->My, owned);
+  if (rc) return rc;
+  const size_t M = (size_t)P->Mx * P->My;
+  double* field = nullptr;
+  cplx<double>* S = nullptr;
+  double* Gd = nullptr;
+  const size_t gn = (size_t)(ex.g.N2 + 1) * P->Mx;
+  if ((rc = dev_alloc(ctx, (void**)&field, M * sizeof(double)))) return rc;
+  if ((rc = dev_alloc(ctx, (void**)&S, (M / 2) * sizeof(cplx<double>)))) return rc;
+  if ((rc = dev_alloc(ctx, (void**)&Gd, gn * sizeof(double)))) return rc;
+  hipStream_t st = ctx->stream;
+  const double dx = P->d.sx / P->d.nx, dy = P->d.sy / P->d.ny;
+  hipLaunchKernelGGL(k_love_kernel, dim3(1024), dim3(256), 0, st, P->d.nx, P->d.ny, dx, dy, P->d.young, field);
+  InstArr<double> ia{};
+  ia.arr = nullptr;
+  ia.one.src = field;
+  ia.one.S = S;
+  launch_r2c<double, ProdCopy<double>>(ex, ia, 1, 0, st);
+  launch_col<double, 1>(ex, ia, 1, 0, 0, st);
+  hipLaunchKernelGGL(k_unpack_green, dim3(1024), dim3(256), 0, st, ex.g, S, Gd);
+  G.resize(gn);
+  HIPCHK(ctx, hipMemcpyAsync(G.data(), Gd, gn * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  dev_free(ctx, field, M * sizeof(double));
+  dev_free(ctx, S, (M / 2) * sizeof(cplx<double>));
+  dev_free(ctx, Gd, gn * sizeof(double));
+  dev_free(ctx, ex.tw_row, 0);
+  dev_free(ctx, ex.tw_col, 0);
+  dev_free(ctx, ex.tw_post, 0);
+  ctx->used -= std::min(ctx->used, owned);
+  return 0;
+}
+
+template <typename T>
+int upload_kernels(cm_plan* P, const std::vector<double>& Graw) {
+  cm_ctx* ctx = P->ctx;
+  const size_t n = Graw.size();
+  const double M = (double)P->Mx * P->My;
+  std::vector<T> Gs(n), Ks(n);
+  P->G_host.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    const double gv = Graw[i];
+    const double kv = gv != 0.0 ? 1.0 / gv : 0.0;  // FFTElasticHalfSpace.py:563-571
+    const double gs = gv * (-1.0 / (P->area_per_pt * M));
+    P->G_host[i] = gs;
+    Gs[i] = (T)gs;
+    Ks[i] = (T)(kv * (-P->area_per_pt / M));
+  }
+  int rc;
+  if ((rc = dev_alloc(ctx, &P->G, n * sizeof(T)))) return rc;
+  if ((rc = dev_alloc(ctx, &P->K, n * sizeof(T)))) return rc;
+  P->bytes_owned += 2 * n * sizeof(T);
+  HIPCHK(ctx, hipMemcpy(P->G, Gs.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(P->K, Ks.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+template <typename T>
+int plan_build(cm_plan* P) {
+  cm_ctx* ctx = P->ctx;
+  int rc = exec_setup<T>(ctx, P->ex, P->d.nx, P->d.ny, P->Mx, P->My, P->bytes_owned);
+  if (rc) return rc;
+  std::vector<double> Graw;
+  if (P->d.bc == CM_PERIODIC)
+    periodic_green_host(P->d, P->Mx, P->My, Graw);
+  else if ((rc = free_green_device(P, Graw)))
+    return rc;
+  if ((rc = upload_kernels<T>(P, Graw))) return rc;
+  const size_t Sb = (size_t)P->Mx * (P->My / 2) * sizeof(cplx<T>);
+  if ((rc = dev_alloc(ctx, &P->S, Sb))) return rc;
+  P->bytes_owned += Sb;
+  const size_t M = (size_t)P->Mx * P->My;
+  P->nblk_elem = (int)std::max<size_t>(1, std::min<size_t>(2048, (M + 2047) / 2048));
+  return 0;
+}
+
+template <typename T>
+int evaluate(cm_plan* P, const void* in, void* out, int useK, void* stream) {
+  hipStream_t st = stream ? (hipStream_t)stream : P->ctx->stream;
+  InstArr<T> ia{};
+  ia.arr = nullptr;
+  ia.one.src = (const T*)in;
+  ia.one.dst = (T*)out;
+  ia.one.S = (cplx<T>*)P->S;
+  ia.one.G = (const T*)P->G;
+  ia.one.K = (const T*)P->K;
+  launch_r2c<T, ProdCopy<T>>(P->ex, ia, 1, 0, st);
+  launch_col<T, 0>(P->ex, ia, 1, useK, 0, st);
+  launch_c2r<T, ConsCopy<T>>(P->ex, ia, 1, 0, st);
+  HIPCHK(P->ctx, hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// CCG driver
+
+struct WsLayout {
+  size_t t, S, sc, part, ctr, inst, inst0, total;
+  size_t st_t, st_S, st_sc, st_part, st_ctr;  // per-instance strides
+};
+
+template <typename T>
+WsLayout ws_layout(const cm_plan* P, int B) {
+  const size_t M = (size_t)P->Mx * P->My;
+  const size_t nparts = std::max<size_t>((size_t)P->nblk_elem, (size_t)(P->Mx / P->ex.g.R)) * 8;
+  auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  WsLayout L{};
+  L.st_t = al(M * sizeof(T));
+  L.st_S = al((M / 2) * sizeof(cplx<T>));
+  L.st_sc = al(sizeof(Scal));
+  L.st_part = al(nparts * sizeof(double));
+  L.st_ctr = al(8 * sizeof(unsigned));
+  size_t o = 0;
+  L.t = o; o += L.st_t * B;
+  L.S = o; o += L.st_S * B;
+  L.sc = o; o += L.st_sc * B;
+  L.part = o; o += L.st_part * B;
+  L.ctr = o; o += L.st_ctr * B;
+  L.inst = o; o += al(sizeof(Inst<T>) * B);
+  L.inst0 = o; o += al(sizeof(Inst<T>) * B);
+  L.total = o;
+  return L;
+}
+
+// Workspace for B instances: conjugate directions, spectra, scalar state,
+// partial reductions, tickets and the device instance tables.
+template <typename T>
+int ensure_ws(cm_plan* P, int B, WsLayout& L) {
+  if (P->ws && (size_t)B <= P->ws_B) {
+    L = ws_layout<T>(P, (int)P->ws_B);
+    return 0;
+  }
+  cm_ctx* ctx = P->ctx;
+  if (P->ws) {
+    (void)hipStreamSynchronize(ctx->stream);
+    dev_free(ctx, P->ws, P->ws_bytes);
+    P->ws = nullptr;
+    P->ws_bytes = 0;
+    P->ws_B = 0;
+  }
+  L = ws_layout<T>(P, B);
+  int rc = dev_alloc(ctx, &P->ws, L.total);
+  if (rc) return rc;
+  P->ws_bytes = L.total;
+  P->ws_B = B;
+  return 0;
+}
+
+void fill_stats(const Scal& s, cm_ccg_stats* o) {
+  o->status = s.done ? s.status : CM_RUNNING;
+  o->nit = s.it;
+  o->nfev = s.nfev;
+  o->last_mode = s.delta_str;
+  o->offset = s.offset;
+  o->fun = s.fun;
+  o->area = s.A_contact;
+  o->cg_area = s.A_cg;
+  o->plastic_area = s.A_fl;
+  o->total_force = s.total_force;
+  o->rms_pen = s.rms_pen;
+  o->max_pen = s.max_pen;
+  o->max_pres = s.max_pres;
+  o->pad_pres = s.pad_pres;
+  o->maxdu = s.maxdu;
+  o->tau = s.tau;
+  o->mixfac = s.mix;
+  o->nc_count = s.nc_count;
+}
+
+// Pinned host staging that lives until the solve has synchronised.
+template <typename X>
+struct Pinned {
+  X* p = nullptr;
+  explicit Pinned(size_t n) {
+    if (hipHostMalloc((void**)&p, sizeof(X) * (n ? n : 1), 0) != hipSuccess) p = nullptr;
+  }
+  ~Pinned() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
+template <typename T>
+int solve_impl(cm_plan* P, int B, const cm_ccg_params* prm, const double* const* hs,
+               const uint8_t* const* surfs, void* const* us, void* const* fs, uint8_t* const* cms,
+               cm_ccg_stats* stats, cm_progress_fn progress, void* user, void* stream) {
+  cm_ctx* ctx = P->ctx;
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  WsLayout L;
+  int rc = ensure_ws<T>(P, B, L);
+  if (rc) return rc;
+  char* base = (char*)P->ws;
+  const size_t M = (size_t)P->Mx * P->My;
+  Pinned<Inst<T>> hin(B), hin0(B);
+  Pinned<Scal> hsc(B), hout(B);
+  if (!hin.p || !hin0.p || !hsc.p || !hout.p) return fail(ctx, CM_E_OOM, "pinned host allocation failed");
+  bool any_bisect = false;
+  int maxiter = 0;
+  const int ips = prm[0].iters_per_sync;
+  int graph = prm[0].use_graph;
+  for (int b = 0; b < B; ++b) {
+    const cm_ccg_params& q = prm[b];
+    Inst<T>& I = hin.p[b];
+    std::memset(&I, 0, sizeof(I));
+    I.u = (T*)us[b];
+    I.f = (T*)fs[b];
+    I.t = (T*)(base + L.t + b * L.st_t);
+    I.cm = cms[b];
+    I.h = hs[b];
+    I.surf = surfs ? surfs[b] : nullptr;
+    I.S = (cplx<T>*)(base + L.S + b * L.st_S);
+    I.G = (const T*)P->G;
+    I.K = (const T*)P->K;
+    I.sc = (Scal*)(base + L.sc + b * L.st_sc);
+    I.part = (double*)(base + L.part + b * L.st_part);
+    I.ctr = (unsigned*)(base + L.ctr + b * L.st_ctr);
+    // set-up table: the initial transform reads u (forces from displacements)
+    // or f (displacements from initial forces) and writes the other field
+    hin0.p[b] = I;
+    if (q.init_mode == 0) {
+      hin0.p[b].src = I.u;
+    } else {
+      hin0.p[b].src = I.f;
+      hin0.p[b].dst = I.u;
+    }
+    Scal& s = hsc.p[b];
+    std::memset(&s, 0, sizeof(Scal));
+    s.F = q.external_force;
+    s.H = q.hardness;
+    s.pentol = q.pentol;
+    s.forcetol = q.forcetol;
+    s.thermotol = q.thermotol;
+    s.mixfac = q.mixfac;
+    s.mixdecfac = q.mixdecfac;
+    s.nb_surface_pts = (double)P->d.nx * P->d.ny;
+    s.n_pad = (double)M;  // ElemInit subtracts the number of computational points
+    s.offset = std::isnan(q.offset) ? 0.0 : q.offset;  // offset=None -> 0 (:167-168)
+    s.shift = s.offset;
+    s.G_old = 1.0;  // :224
+    s.mix = q.mixfac;
+    s.load_ctl = q.load_control;
+    s.has_H = q.has_hardness;
+    s.H_truthy = q.has_hardness && q.hardness != 0.0;
+    s.minmixsteps = q.minmixsteps;
+    s.maxiter = q.maxiter;
+    s.it = 0;
+    s.nfev = 1;  // the initial evaluation (:217)
+    s.delta_str = DS_RESET;
+    if (q.maxiter < 1) {
+      s.done = 1;
+      s.status = CM_MAXITER;
+    }
+    any_bisect = any_bisect || (q.has_hardness && q.load_control);
+    maxiter = std::max(maxiter, q.maxiter);
+  }
+  Inst<T>* dinst = (Inst<T>*)(base + L.inst);
+  Inst<T>* dinst0 = (Inst<T>*)(base + L.inst0);
+  for (int b = 0; b < B; ++b)
+    HIPCHK(ctx, hipMemcpyAsync(hin.p[b].sc, &hsc.p[b], sizeof(Scal), hipMemcpyHostToDevice, st));
+  HIPCHK(ctx, hipMemcpyAsync(dinst, hin.p, sizeof(Inst<T>) * B, hipMemcpyHostToDevice, st));
+  HIPCHK(ctx, hipMemcpyAsync(dinst0, hin0.p, sizeof(Inst<T>) * B, hipMemcpyHostToDevice, st));
+  for (int b = 0; b < B; ++b) {
+    HIPCHK(ctx, hipMemsetAsync(hin.p[b].ctr, 0, 8 * sizeof(unsigned), st));
+    HIPCHK(ctx, hipMemsetAsync(hin.p[b].t, 0, M * sizeof(T), st));
+  }
+  InstArr<T> ia{};
+  ia.arr = dinst;
+  InstArr<T> ia0{};
+  ia0.arr = dinst0;
+  const FftExec& ex = P->ex;
+  const int nbe = P->nblk_elem;
+  // ---- set-up (:179-219)
+  launch_elem<T, ElemInit<T>>(ex, nbe, ia, B, 0, st);
+  if (prm[0].init_mode == 0) {
+    launch_r2c<T, ProdCopy<T>>(ex, ia0, B, 1, st);  // f = evaluate_force(u), f[pad] = 0
+    launch_col<T, 0>(ex, ia0, B, 1, 0, st);
+    launch_c2r<T, ConsInitForce<T>>(ex, ia0, B, 1, st);
+  } else {
+    launch_r2c<T, ProdCopy<T>>(ex, ia0, B, 1, st);  // u = evaluate_disp(f)
+    launch_col<T, 0>(ex, ia0, B, 0, 0, st);
+    launch_c2r<T, ConsCopy<T>>(ex, ia0, B, 1, st);
+    launch_elem<T, ElemZeroPad<T>>(ex, nbe, ia, B, 1, st);
+  }
+  HIPCHK(ctx, hipGetLastError());
+
+  auto enqueue_iteration = [&]() {
+    launch_elem<T, ElemPhaseA<T>>(ex, nbe, ia, B, 2, st);
+    if (any_bisect) hipLaunchKernelGGL((k_bisect<T>), dim3(B), dim3(1024), 0, st, ex.g, ia);
+    launch_r2c<T, ProdDir<T>>(ex, ia, B, 3, st);
+    launch_col<T, 0>(ex, ia, B, 0, 2, st);
+    launch_c2r<T, ConsTau<T>>(ex, ia, B, 4, st);
+    launch_elem<T, ElemPhaseB<T>>(ex, nbe, ia, B, 5, st);
+    launch_r2c<T, ProdForce<T>>(ex, ia, B, 6, st);
+    launch_col<T, 0>(ex, ia, B, 0, 1, st);
+    launch_c2r<T, ConsDisp<T>>(ex, ia, B, 7, st);
+  };
+
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graphh = nullptr;
+  int chunk = ips > 0 ? ips : 4;
+  int graph_chunk = 0;
+  long long issued = 0;
+  int result = 0;
+  while (true) {
+    bool launched = false;
+    if (graph && ips <= 0 && chunk >= 16) {
+      if (!gexec || graph_chunk != chunk) {
+        if (gexec) {
+          (void)hipGraphExecDestroy(gexec);
+          (void)hipGraphDestroy(graphh);
+          gexec = nullptr;
+          graphh = nullptr;
+        }
+        hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+          for (int k = 0; k < chunk; ++k) enqueue_iteration();
+          e = hipStreamEndCapture(st, &graphh);
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&gexec, graphh, nullptr, nullptr, 0);
+        if (e == hipSuccess) {
+          graph_chunk = chunk;
+        } else {
+          graph = 0;
+          gexec = nullptr;
+          (void)hipGetLastError();
+        }
+      }
+      if (gexec) {
+        HIPCHK(ctx, hipGraphLaunch(gexec, st));
+        launched = true;
+      }
+    }
+    if (!launched)
+      for (int k = 0; k < chunk; ++k) enqueue_iteration();
+    issued += chunk;
+    for (int b = 0; b < B; ++b)
+      (void)hipMemcpyAsync(&hout.p[b], hin.p[b].sc, sizeof(Scal), hipMemcpyDeviceToHost, st);
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+      result = fail(ctx, CM_E_HIP, std::string("solve: ") + hipGetErrorString(e));
+      break;
+    }
+    if (progress) {
+      cm_ccg_stats tmp{};
+      fill_stats(hout.p[0], &tmp);
+      progress(user, &tmp);
+    }
+    bool all_done = true;
+    for (int b = 0; b < B; ++b) all_done = all_done && hout.p[b].done;
+    if (all_done) break;
+    if (issued > (long long)maxiter + 8) {
+      result = fail(ctx, CM_E_ARG, "solver did not terminate within maxiter");
+      break;
+    }
+    if (ips <= 0) chunk = std::min(chunk * 2, 64);
+  }
+  if (gexec) {
+    (void)hipGraphExecDestroy(gexec);
+    (void)hipGraphDestroy(graphh);
+  }
+  if (result == 0) {
+    launch_elem<T, ElemEnergy<T>>(ex, nbe, ia, B, 0, st);
+    for (int b = 0; b < B; ++b)
+      (void)hipMemcpyAsync(&hout.p[b], hin.p[b].sc, sizeof(Scal), hipMemcpyDeviceToHost, st);
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) result = fail(ctx, CM_E_HIP, std::string("finalize: ") + hipGetErrorString(e));
+  }
+  if (result == 0) {
+    for (int b = 0; b < B; ++b) fill_stats(hout.p[b], &stats[b]);
+    result = stats[0].status;
+    for (int b = 1; b < B; ++b)
+      if (stats[b].status < 0) result = stats[b].status;
+  }
+  return result;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+
+extern "C" {
+
+const char* cm_version(void) { return "cmhip 0.1 (gfx950)"; }
+
+const char* cm_last_error(cm_ctx* ctx) {
+  if (!ctx) return g_err_noctx.c_str();
+  return ctx->err.c_str();
+}
+
+int cm_ctx_create(int device, size_t mem_budget, cm_ctx** out) {
+  if (!out) return fail(nullptr, CM_E_ARG, "out is null");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(nullptr, CM_E_HIP, "no HIP device available");
+  if (device < 0 || device >= n) return fail(nullptr, CM_E_ARG, "bad device index");
+  if (hipSetDevice(device) != hipSuccess) return fail(nullptr, CM_E_HIP, "hipSetDevice failed");
+  cm_ctx* c = new cm_ctx();
+  c->device = device;
+  c->budget = mem_budget;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(nullptr, CM_E_HIP, "hipStreamCreate failed");
+  }
+  *out = c;
+  return 0;
+}
+
+int cm_ctx_destroy(cm_ctx* ctx) {
+  if (!ctx) return 0;
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return 0;
+}
+
+size_t cm_ctx_mem_used(cm_ctx* ctx) { return ctx ? ctx->used : 0; }
+
+int cm_ctx_stream(cm_ctx* ctx, void** stream) {
+  if (!ctx || !stream) return fail(ctx, CM_E_ARG, "null argument");
+  *stream = (void*)ctx->stream;
+  return 0;
+}
+
+int cm_plan_create(cm_ctx* ctx, const cm_substrate_desc* d, cm_plan** out) {
+  if (!ctx || !d || !out) return fail(ctx, CM_E_ARG, "null argument");
+  if (d->nx < 1 || d->ny < 1) return fail(ctx, CM_E_SHAPE, "grid must be at least 1x1");
+  if (!(d->sx > 0) || !(d->sy > 0)) return fail(ctx, CM_E_ARG, "physical sizes must be positive");
+  if (d->bc != CM_PERIODIC && d->bc != CM_FREE) return fail(ctx, CM_E_ARG, "bc must be CM_PERIODIC or CM_FREE");
+  if (d->precision != CM_FP64 && d->precision != CM_FP32) return fail(ctx, CM_E_ARG, "bad precision");
+  (void)hipSetDevice(ctx->device);
+  cm_plan* P = new cm_plan();
+  P->ctx = ctx;
+  P->d = *d;
+  P->prec = d->precision;
+  P->Mx = d->bc == CM_FREE ? 2 * d->nx : d->nx;
+  P->My = d->bc == CM_FREE ? 2 * d->ny : d->ny;
+  P->area_per_pt = d->sx * d->sy / ((double)d->nx * d->ny);
+  if (P->My % 2 != 0) {
+    delete P;
+    return fail(ctx, CM_E_UNSUPPORTED, "the contiguous grid dimension My must be even");
+  }
+  int rc = P->prec == CM_FP64 ? plan_build<double>(P) : plan_build<float>(P);
+  if (rc) {
+    plan_free(P);
+    return rc;
+  }
+  *out = P;
+  return 0;
+}
+
+int cm_plan_destroy(cm_plan* plan) {
+  if (!plan) return 0;
+  (void)hipStreamSynchronize(plan->ctx->stream);
+  return plan_free(plan);
+}
+
+int cm_plan_grid(cm_plan* plan, int* Mx, int* My) {
+  if (!plan) return fail(nullptr, CM_E_ARG, "null plan");
+  if (Mx) *Mx = plan->Mx;
+  if (My) *My = plan->My;
+  return 0;
+}
+
+int cm_plan_green_host(cm_plan* plan, double* out) {
+  if (!plan || !out) return fail(nullptr, CM_E_ARG, "null argument");
+  std::memcpy(out, plan->G_host.data(), plan->G_host.size() * sizeof(double));
+  return 0;
+}
+
+int cm_evaluate_disp(cm_plan* plan, const void* f, void* u, void* stream) {
+  if (!plan || !f || !u) return fail(plan ? plan->ctx : nullptr, CM_E_ARG, "null argument");
+  return plan->prec == CM_FP64 ? evaluate<double>(plan, f, u, 0, stream) : evaluate<float>(plan, f, u, 0, stream);
+}
+
+int cm_evaluate_force(cm_plan* plan, const void* u, void* f, void* stream) {
+  if (!plan || !f || !u) return fail(plan ? plan->ctx : nullptr, CM_E_ARG, "null argument");
+  return plan->prec == CM_FP64 ? evaluate<double>(plan, u, f, 1, stream) : evaluate<float>(plan, u, f, 1, stream);
+}
+
+int cm_ccg_solve(cm_plan* plan, const cm_ccg_params* params, const double* heights, const uint8_t* surf_mask,
+                 void* u, void* f, uint8_t* cmask, cm_ccg_stats* stats, cm_progress_fn progress, void* user,
+                 void* stream) {
+  if (!plan || !params || !heights || !u || !f || !cmask || !stats)
+    return fail(plan ? plan->ctx : nullptr, CM_E_ARG, "null argument");
+  const double* hs[1] = {heights};
+  const uint8_t* ms[1] = {surf_mask};
+  void* us[1] = {u};
+  void* fs[1] = {f};
+  uint8_t* cs[1] = {cmask};
+  return plan->prec == CM_FP64
+             ? solve_impl<double>(plan, 1, params, hs, ms, us, fs, cs, stats, progress, user, stream)
+             : solve_impl<float>(plan, 1, params, hs, ms, us, fs, cs, stats, progress, user, stream);
+}
+
+int cm_ccg_solve_batch(cm_plan* plan, int B, const cm_ccg_params* params, const double* const* heights,
+                       const uint8_t* const* surf_masks, void* const* u, void* const* f, uint8_t* const* cmask,
+                       cm_ccg_stats* stats, void* stream) {
+  if (!plan || B < 1 || !params || !heights || !u || !f || !cmask || !stats)
+    return fail(plan ? plan->ctx : nullptr, CM_E_ARG, "bad argument");
+  for (int b = 1; b < B; ++b)
+    if (params[b].init_mode != params[0].init_mode)
+      return fail(plan->ctx, CM_E_ARG, "all instances of a batch must share init_mode");
+  return plan->prec == CM_FP64
+             ? solve_impl<double>(plan, B, params, heights, surf_masks, u, f, cmask, stats, nullptr, nullptr, stream)
+             : solve_impl<float>(plan, B, params, heights, surf_masks, u, f, cmask, stats, nullptr, nullptr, stream);
+}
+
+}  // extern "C"

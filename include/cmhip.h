@@ -1,0 +1,154 @@
+/*
+ * cmhip.h -- C ABI of the gfx950 contact solver (libcmhip.so).
+ *
+ * Plain C: device pointers are `void*` (HIP device memory, e.g. the data
+ * pointer of a torch-ROCm tensor), sizes are ints, the stream is a
+ * `hipStream_t` passed as `void*` (NULL = the context's own stream).  No torch
+ * types appear here.  Every entry point returns an `int` status: 0 (or
+ * CM_CONVERGED) on success, a negative CM_E_* code on error; the message of the
+ * last error is available from cm_last_error().
+ *
+ * Array layout: real fields on the computational grid are C-order (Mx, My)
+ * with My contiguous: Mx = nx, My = ny for periodic substrates and
+ * Mx = 2 nx, My = 2 ny for free ones (the topography occupies the top-left
+ * nx x ny block).  Heights are C-order (nx, ny) FP64.  Forces are per-pixel
+ * forces (not pressures) with compressive force negative, as in the reference.
+ *
+ * Precision: CM_FP64 keeps every field and transform in double.  CM_FP32
+ * keeps fields/spectra in float while all global reductions, the CG scalars
+ * and the gap arithmetic run in double.
+ */
+#ifndef CMHIP_H
+#define CMHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ----------------------------------------------------- */
+#define CM_OK 0
+#define CM_CONVERGED 0
+#define CM_MAXITER 1          /* reference: success=False, "Reached maxiter = N" */
+#define CM_RUNNING 2          /* progress callback only: solve still iterating */
+#define CM_E_ARG (-1)         /* bad argument (reference: ValueError) */
+#define CM_E_SHAPE (-2)       /* shape/size mismatch (reference: Substrate.Error) */
+#define CM_E_NAN (-3)         /* reference: RuntimeError('nan encountered.') */
+#define CM_E_OOM (-4)         /* device memory budget exceeded */
+#define CM_E_HIP (-5)         /* HIP runtime error */
+#define CM_E_UNSUPPORTED (-6) /* grid size not supported by the FFT (needs 2^a 3^b 5^c) */
+#define CM_E_BISECT (-7)      /* bearing-area bisection failed (reference: scipy bisect raising) */
+
+#define CM_PERIODIC 0
+#define CM_FREE 1
+#define CM_FP64 0
+#define CM_FP32 1
+
+typedef struct cm_ctx cm_ctx;
+typedef struct cm_plan cm_plan;
+
+/* Substrate description.  Replaces the constructor arguments of
+ * PeriodicFFTElasticHalfSpace (FFTElasticHalfSpace.py:169-306) and
+ * FreeFFTElasticHalfSpace (FFTElasticHalfSpace.py:912-964). */
+typedef struct cm_substrate_desc {
+  int nx, ny;              /* topography grid */
+  int bc;                  /* CM_PERIODIC or CM_FREE */
+  int precision;           /* CM_FP64 or CM_FP32 */
+  double sx, sy;           /* physical sizes */
+  double young;            /* Young's modulus (contact modulus if poisson == 0) */
+  double poisson;          /* Poisson ratio (periodic only) */
+  int q0_mode;             /* periodic: 0 = stiffness_q0 None, 1 = stiffness_q0 == 0, 2 = given value */
+  double stiffness_q0;     /* used when q0_mode == 2 */
+  double thickness;        /* periodic finite thickness; <= 0 or NaN = infinite half-space */
+} cm_substrate_desc;
+
+/* Keyword arguments of constrained_conjugate_gradients
+ * (ConstrainedConjugateGradients.py:47-60). */
+typedef struct cm_ccg_params {
+  int load_control;        /* 1: external_force given, 0: offset given */
+  double external_force;
+  double offset;
+  int has_hardness;        /* hardness is not None */
+  double hardness;         /* per-pixel force cap (pressure * area_per_pt) */
+  double pentol, forcetol, thermotol;
+  double mixfac, mixdecfac;
+  int minmixsteps;
+  int maxiter;
+  int init_mode;           /* 0: forces from initial displacements, 1: initial forces given */
+  int iters_per_sync;      /* 0 = adaptive; 1 = host sees every iteration */
+  int use_graph;           /* 1 = replay chunks of iterations from a captured hipGraph */
+} cm_ccg_params;
+
+/* Per-solve statistics; the scalar fields of the reference OptimizeResult
+ * and of its per-iteration log (ConstrainedConjugateGradients.py:436-509). */
+typedef struct cm_ccg_stats {
+  int status;              /* CM_CONVERGED, CM_MAXITER or CM_E_* */
+  int nit, nfev;
+  int last_mode;           /* 0 = 'reset', 1 = 'sd', 2 = 'cg', 3 = 'mix' (delta_str) */
+  double offset, fun;
+  double area, cg_area, plastic_area;
+  double total_force, rms_pen, max_pen, max_pres, pad_pres, maxdu, tau, mixfac;
+  double nc_count;
+} cm_ccg_stats;
+
+/* Called by cm_ccg_solve after every host synchronisation (see iters_per_sync). */
+typedef void (*cm_progress_fn)(void* user, const cm_ccg_stats* stats);
+
+const char* cm_last_error(cm_ctx* ctx);
+const char* cm_version(void);
+
+/* Context: device, default stream, memory budget (bytes, 0 = unlimited). */
+int cm_ctx_create(int device, size_t mem_budget, cm_ctx** out);
+int cm_ctx_destroy(cm_ctx* ctx);
+size_t cm_ctx_mem_used(cm_ctx* ctx);
+/* The context's own non-blocking stream (what a NULL stream argument means). */
+int cm_ctx_stream(cm_ctx* ctx, void** stream);
+
+/* Plan: Green's function, FFT twiddles and launch shapes for one substrate.
+ * Replaces PeriodicFFTElasticHalfSpace._compute_greens_function /
+ * _compute_surface_stiffness (FFTElasticHalfSpace.py:445-571) and
+ * FreeFFTElasticHalfSpace._compute_greens_function (:1033-1139). */
+int cm_plan_create(cm_ctx* ctx, const cm_substrate_desc* desc, cm_plan** out);
+int cm_plan_destroy(cm_plan* plan);
+/* Computational grid (Mx, My) of the plan. */
+int cm_plan_grid(cm_plan* plan, int* Mx, int* My);
+/* Copy the plan's real Green's function (FP64, C-order (My/2+1, Mx), already
+ * including the -1/(area_per_pt * Mx * My) factor of evaluate_disp) to host. */
+int cm_plan_green_host(cm_plan* plan, double* host_out);
+
+/* u = evaluate_disp(f) on the (Mx, My) grid (FFTElasticHalfSpace.py:573-589,
+ * padded branch of 1141-1164).  f, u: device arrays of the plan precision;
+ * in-place (u == f) is allowed. */
+int cm_evaluate_disp(cm_plan* plan, const void* f, void* u, void* stream);
+/* f = evaluate_force(u) (FFTElasticHalfSpace.py:591-611). */
+int cm_evaluate_force(cm_plan* plan, const void* u, void* f, void* stream);
+
+/* Constrained conjugate gradients (ConstrainedConjugateGradients.py:47-509).
+ *  heights   device FP64 (nx, ny)
+ *  surf_mask device uint8 (nx, ny), 1 = valid pixel, NULL = no mask
+ *  u         device (Mx, My) plan precision: in = initial displacements
+ *            (zeros if none), out = result.x
+ *  f         device (Mx, My): in = initial forces when init_mode == 1,
+ *            out = final forces (result.jac = -f[:nx, :ny])
+ *  cmask     device uint8 (Mx, My) out: bit0 = result.active_set,
+ *            bit1 = gap < 0 at the start of the last iteration (result.plastic,
+ *            and the active set on the topography after a mixing step)
+ * Returns CM_CONVERGED, CM_MAXITER or a negative error. */
+int cm_ccg_solve(cm_plan* plan, const cm_ccg_params* params, const double* heights,
+                 const uint8_t* surf_mask, void* u, void* f, uint8_t* cmask,
+                 cm_ccg_stats* stats, cm_progress_fn progress, void* user, void* stream);
+
+/* Batched solve of B independent instances on the same plan (same grid and
+ * elastic constants), one launch per kernel for all instances.  Arrays are
+ * host arrays of B device pointers; stats is a host array of B records. */
+int cm_ccg_solve_batch(cm_plan* plan, int B, const cm_ccg_params* params,
+                       const double* const* heights, const uint8_t* const* surf_masks,
+                       void* const* u, void* const* f, uint8_t* const* cmask,
+                       cm_ccg_stats* stats, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CMHIP_H */
