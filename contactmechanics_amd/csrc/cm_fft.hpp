@@ -115,7 +115,7 @@ template <typename T, bool INV> struct Dft<T, 8, INV> {
 struct LinePlan {
   int L;          // transform length
   int tpl;        // threads per line
-  int E;          // elements per thread (L / tpl)
+  int E;          // most elements a thread holds in any pass
   int npass;      // number of radix passes
   int radix[14];  // radix of each pass (product == L)
 };
@@ -168,7 +168,8 @@ __device__ void fft_line(cplx<T>* lds, int base, int stride, const LinePlan& p, 
   int Ns = 1;
   for (int s = 0; s < p.npass; ++s) {
     const int R = p.radix[s];
-    const int nb = active ? p.E / R : 0;
+    const int nbf = p.L / R;  // butterflies of this pass, dealt round-robin to threads
+    const int nb = (active && tl < nbf) ? (nbf - tl + p.tpl - 1) / p.tpl : 0;
     switch (R) {
       case 2: stockham_pass<T, 2, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
       case 3: stockham_pass<T, 3, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;

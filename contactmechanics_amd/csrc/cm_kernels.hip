@@ -420,11 +420,9 @@ struct ElemPhaseA {
     in.cm[p] = c ? 1 : 0;
     if (is_comp(g, in.surf, i, j)) {
       const double g0 = (double)in.u[p] - in.h[i * g.ny + j];
-      if (c) {
-        const double d = g0 - s->shift;
-        acc[2] += d;
-        acc[3] += d * d;
-      }
+      const double d = g0 - s->shift;
+      if (c) acc[2] += d;  // sum over the contact set only (:269)
+      acc[3] += (c ? 1.0 : 0.0) * d * d;  // c * g * g: a NaN gap poisons G as in :273
       acc[4] = red_apply(RMAX, acc[4], g0);
       acc[5] = red_apply(RMIN, acc[5], g0);
     }
@@ -443,7 +441,7 @@ struct ElemPhaseA {
       if (s->load_ctl) s->offset = a[1] > 0 ? s->shift + a[2] / a[1] : 0.0;  // :266-269
       const double m = s->offset - s->shift;
       double G = a[3] - 2.0 * m * a[2] + a[1] * m * m;  // sum_c (g - offset)^2 (:273)
-      s->G = G > 0.0 ? G : 0.0;
+      s->G = G < 0.0 ? 0.0 : G;  // clamp round-off below zero, keep NaN
       s->use_beta = (s->delta > 0 && s->G_old > 0) ? 1 : 0;  // :276
       s->beta = s->use_beta ? s->delta * (s->G / s->G_old) : 0.0;
     } else {
@@ -875,47 +873,54 @@ void dev_free(cm_ctx* ctx, void* p, size_t bytes) {
   }
 }
 
-// Choose radices and threads-per-line for a length L: every radix must divide
-// the elements-per-thread E so each thread runs whole butterflies.  Fewer
-// passes first, then E close to `e_pref`.
-bool make_line_plan(int L, int max_tpl, int e_pref, LinePlan& best) {
-  static const int Es[] = {16, 32, 12, 24, 20, 8, 10, 6, 4, 2};
-  bool found = false;
-  int best_score = 1 << 30;
-  for (int E : Es) {
-    if (L % E != 0) continue;
-    const int tpl = L / E;
-    if (tpl > max_tpl) continue;
-    LinePlan lp{};
-    lp.L = L;
-    lp.E = E;
-    lp.tpl = tpl;
-    int rem = L, np = 0;
-    bool ok = true;
-    while (rem > 1) {
-      int pick = 0;
-      for (int R : {8, 4, 5, 3, 2})
-        if (E % R == 0 && rem % R == 0) {
-          pick = R;
-          break;
-        }
-      if (!pick || np >= 14) {
-        ok = false;
+// Radix sequence and threads-per-line for a length L = 2^a 3^b 5^c.
+// Radices 8, 4, 5, 3, 2 (largest first).  A pass of radix R has L/R
+// butterflies dealt round-robin to tpl threads; a thread holds at most
+// ceil(L/R/tpl)*R elements, which must fit the kernel's register budget emax.
+// Prefer tpl dividing every L/R (balanced passes), then larger tpl.
+bool make_line_plan(int L, int max_tpl, int emax, LinePlan& best) {
+  if (L < 2) return false;
+  LinePlan lp{};
+  lp.L = L;
+  int rem = L, np = 0;
+  while (rem > 1) {
+    int pick = 0;
+    for (int R : {8, 4, 5, 3, 2})
+      if (rem % R == 0) {
+        pick = R;
         break;
       }
-      lp.radix[np++] = pick;
-      rem /= pick;
-    }
-    if (!ok) continue;
-    lp.npass = np;
-    const int score = np * 100 + std::abs(E - e_pref);
-    if (score < best_score) {
-      best_score = score;
-      best = lp;
-      found = true;
+    if (!pick || np >= 14) return false;
+    lp.radix[np++] = pick;
+    rem /= pick;
+  }
+  lp.npass = np;
+  int need = 1;  // smallest tpl meeting the register budget
+  for (int s = 0; s < np; ++s) {
+    const int R = lp.radix[s];
+    const int nbmax = emax / R;
+    if (nbmax < 1) return false;
+    need = std::max(need, (L / R + nbmax - 1) / nbmax);
+  }
+  if (need > max_tpl) return false;
+  int tpl = need;
+  for (int t = need; t <= max_tpl && t <= L; ++t) {
+    bool balanced = true;
+    for (int s = 0; s < np; ++s) balanced = balanced && (L / lp.radix[s]) % t == 0;
+    if (balanced) {
+      tpl = t;
+      break;
     }
   }
-  return found;
+  lp.tpl = tpl;
+  int E = 0;
+  for (int s = 0; s < np; ++s) {
+    const int R = lp.radix[s];
+    E = std::max(E, ((L / R + tpl - 1) / tpl) * R);
+  }
+  lp.E = E;
+  best = lp;
+  return true;
 }
 
 template <typename T>
@@ -982,7 +987,11 @@ int exec_setup(cm_ctx* ctx, FftExec& ex, int nx, int ny, int Mx, int My, size_t&
   g.N2 = My / 2;
   const size_t cs = sizeof(cplx<T>);
   constexpr int kMaxBlock = 256;  // FFT kernels are built with __launch_bounds__(256)
-  if (!make_line_plan(g.N2, kMaxBlock, 16, ex.row) || !make_line_plan(Mx, kMaxBlock, cs == 8 ? 32 : 16, ex.col))
+  // 16 register elements per thread when that fits the block, else 32
+  auto plan_len = [&](int L, LinePlan& lp) {
+    return make_line_plan(L, kMaxBlock, 16, lp) || make_line_plan(L, kMaxBlock, 32, lp);
+  };
+  if (!plan_len(g.N2, ex.row) || !plan_len(Mx, ex.col))
     return fail(ctx, CM_E_UNSUPPORTED,
                 "grid " + std::to_string(Mx) + "x" + std::to_string(My) +
                     " not supported: Mx and My/2 must factor into 2, 3, 5");
