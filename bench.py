@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Constrained-CG benchmark on one MI355X.
+
+Workload (default, --config 2): periodic 2048^2 self-affine surface
+(H = 0.8, RMS slope 0.1, seed 2048), mean pressure 0.01 E*, E* = L = 1.
+
+A "step" is one Polonsky-Keer iteration over the whole grid (two FFT
+convolutions plus the fused projection and reductions).  The timed region
+runs exactly --steps iterations of a resident solve (inputs already in HBM)
+with unreachable tolerances, so no iteration is skipped; the reported value
+is grid points x iterations / seconds.  A separate full solve with the mapped
+tolerances (gtol = 1e-6: pentol = gtol L, forcetol = gtol area_per_pt)
+gives time-to-solution and the pressure error of the FP32 path against the
+FP64 path.  The CPU baseline times the FP64 NumPy oracle for a bounded number
+of iterations of the same workload on the host.
+
+Prints ONE JSON line.
+"""
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def kernel_bytes(name, M, Mt, w):
+    """Bytes each fused kernel must read + write once per launch (FP width w;
+    heights are FP64, contact masks one byte)."""
+    return {
+        "phase_a": M * (2 * w + 1) + 8 * Mt,
+        "bisect": 0,
+        "dir_r2c": M * (4 * w + 1) + 8 * Mt,
+        "col_conv1": int(2.5 * w * M),
+        "tau_c2r": M * (2 * w + 1),
+        "phase_b": M * (3 * w + 1) + 8 * Mt,
+        "force_r2c": M * (5 * w + 2) + 8 * Mt,
+        "col_conv2": int(2.5 * w * M),
+        "disp_c2r": M * (3 * w + 1) + 8 * Mt,
+    }[name]
+
+
+def canonical_iteration_bytes(M, Mt, w, free):
+    """Algorithmic bytes per CCG iteration: w (19 M) periodic, w (19.25 M) free
+    (6 FFT passes x 2M words, state 6M + Mt words, + real G for free)."""
+    return w * (12 * M + 6 * M + Mt + (M // 4 if free else 0))
+
+
+def throughput(sub, heights, F, steps, warmup, precision):
+    import torch
+    from contactmechanics_amd.session import CcgSession, make_params
+    params = make_params(external_force=F, pentol=0.0, forcetol=0.0, thermotol=0.0,
+                         maxiter=warmup + steps + 20, use_graph=False)
+    s = CcgSession(sub, [heights], [params])
+    s.step(max(warmup, 1))
+    s.poll()
+    nit0 = s.stats[0].nit
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.poll()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    nit1 = s.stats[0].nit
+    prof = s.profile(5)
+    s.close()
+    return dict(seconds=t1 - t0, iterations=nit1 - nit0, kernel_us=prof)
+
+
+def full_solve(sub, heights, cfg, maxiter=20000):
+    import torch
+    from contactmechanics_amd import constrained_conjugate_gradients as ccg
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = ccg(sub, heights, external_force=cfg["external_force"], pentol=cfg["pentol"],
+            forcetol=cfg["forcetol"], thermotol=cfg["thermotol"], maxiter=maxiter, return_device=True)
+    torch.cuda.synchronize()
+    return r, time.perf_counter() - t0
+
+
+def rel_errors(p, ref):
+    import torch
+    p = p.to(torch.float64)
+    ref = ref.to(torch.float64)
+    l2 = float(torch.linalg.norm(p - ref) / torch.linalg.norm(ref))
+    linf = float(torch.max(torch.abs(p - ref)) / torch.max(torch.abs(ref)))
+    a, b = int((p > 0).sum()), int((ref > 0).sum())
+    return dict(l2=l2, linf=linf, area_rel=abs(a - b) / max(b, 1))
+
+
+def cpu_baseline(cfg, iters, substrate_gpu):
+    """FP64 NumPy oracle on the same workload: time `iters` iterations."""
+    import numpy as np
+    from oracle.ccg import constrained_conjugate_gradients as ccg_cpu
+    from oracle.halfspace import PeriodicHalfSpace
+    n = cfg["n"]
+    hs = PeriodicHalfSpace((n, n), 1.0, (1.0, 1.0))
+    kw = dict(external_force=cfg["external_force"], pentol=0.0, forcetol=0.0, thermotol=0.0)
+    t0 = time.perf_counter()
+    ccg_cpu(hs, cfg["heights"], maxiter=0, **kw)
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ref = ccg_cpu(hs, cfg["heights"], maxiter=iters, **kw)
+    t_run = time.perf_counter() - t0 - t_setup
+    # same iterations on the GPU (FP64) for a same-seed error figure
+    from contactmechanics_amd import constrained_conjugate_gradients as ccg_gpu
+    g = ccg_gpu(substrate_gpu, cfg["heights"], maxiter=iters, **kw)
+    err = float(np.max(np.abs(g.jac - ref.jac)) / np.max(np.abs(ref.jac)))
+    return dict(seconds=t_run, iterations=iters, gpu_vs_oracle_linf=err)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=2048, help="grid side of the C2 workload")
+    ap.add_argument("--cpu-iters", type=int, default=30, help="oracle iterations for the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(independent solves shard across GPUs, see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from contactmechanics_amd import PeriodicFFTElasticHalfSpace, workloads
+
+    torch.cuda.set_device(0)
+    cfg = workloads.config2(args.n)
+    n = cfg["n"]
+    M = Mt = n * n
+    print(f"# workload: {cfg['name']}", file=sys.stderr, flush=True)
+    sub64 = PeriodicFFTElasticHalfSpace((n, n), 1.0, (1.0, 1.0), precision="fp64")
+    sub32 = PeriodicFFTElasticHalfSpace((n, n), 1.0, (1.0, 1.0), precision="fp32")
+
+    tp64 = throughput(sub64, cfg["heights"], cfg["external_force"], args.steps, args.warmup, "fp64")
+    print(f"# fp64: {tp64['iterations']} its in {tp64['seconds']:.4f}s", file=sys.stderr, flush=True)
+    tp32 = throughput(sub32, cfg["heights"], cfg["external_force"], args.steps, args.warmup, "fp32")
+    print(f"# fp32: {tp32['iterations']} its in {tp32['seconds']:.4f}s", file=sys.stderr, flush=True)
+    assert tp64["iterations"] == args.steps and tp32["iterations"] == args.steps, "iterations were skipped"
+    value = M * args.steps / tp64["seconds"]
+    value32 = M * args.steps / tp32["seconds"]
+
+    # roofline of the dominant kernel (FP64 path), durations measured live
+    ku = tp64["kernel_us"]
+    dom = max((k for k in ku if k != "bisect"), key=lambda k: ku[k])
+    b = kernel_bytes(dom, M, Mt, 8)
+    achieved = b / (ku[dom] * 1e-6) / 1e9
+    it_us = sum(ku.values())
+    iter_gbps = canonical_iteration_bytes(M, Mt, 8, False) / (it_us * 1e-6) / 1e9
+
+    extra = {}
+    if not args.no_solve:
+        r64, t64 = full_solve(sub64, cfg["heights"], cfg)
+        r32, t32 = full_solve(sub32, cfg["heights"], cfg)
+        err = rel_errors(r32.jac, r64.jac)
+        extra["solve"] = dict(
+            fp64=dict(nit=int(r64.nit), seconds=round(t64, 4), success=bool(r64.success)),
+            fp32=dict(nit=int(r32.nit), seconds=round(t32, 4), success=bool(r32.success)),
+            fp32_vs_fp64_pressure_err=err)
+        print(f"# solve fp64 nit={r64.nit} {t64:.3f}s; fp32 nit={r32.nit} {t32:.3f}s; err {err}",
+              file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cb = cpu_baseline(cfg, args.cpu_iters, sub64)
+        cpu = {"value": M * cb["iterations"] / cb["seconds"], "unit": "grid-pt-iters/s", "cores": 1,
+               "kind": "port",
+               "sample": f"{cb['iterations']} iterations of the FP64 NumPy oracle on the same {n}^2 "
+                         f"workload (numpy pocketfft, single thread) in {cb['seconds']:.2f}s",
+               "gpu_fp64_vs_oracle_linf_after_same_iterations": cb["gpu_vs_oracle_linf"],
+               "host": platform.processor() or platform.machine()}
+        print(f"# cpu: {cb}", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "CCG grid-pt-iters/s (FP64 path, rel-err vs FP64 oracle <= 1e-6)",
+        "value": value,
+        "unit": "grid-pt-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": tp64["seconds"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": cfg["name"], "grid": [n, n], "seed": 2048, "external_force": 0.01,
+                   "timed_region": "resident session, exact --steps iterations, tolerances unreachable"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved / 8000.0, "traffic": None,
+                     "bytes_per_launch": b, "avg_us": ku[dom]},
+        "cpu_baseline": cpu,
+        "kernel_us_fp64": {k: round(v, 2) for k, v in ku.items()},
+        "kernel_us_fp32": {k: round(v, 2) for k, v in tp32["kernel_us"].items()},
+        "iteration_gbps_canonical_fp64": iter_gbps,
+        "fp32_path": {"value": value32, "ms_per_step": tp32["seconds"] / args.steps * 1e3,
+                      "speedup_vs_fp64": value32 / value},
+    }
+    out.update(extra)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -99,15 +99,30 @@ SIGNATURES = {
     "cm_ccg_solve_batch": ([_VP, ctypes.c_int, ctypes.POINTER(CcgParams), ctypes.POINTER(_VP),
                             ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(_VP),
                             ctypes.POINTER(_VP), ctypes.POINTER(CcgStats), _VP], ctypes.c_int),
+    "cm_ccg_session_bytes": ([_VP, ctypes.c_int], ctypes.c_size_t),
+    "cm_ccg_begin": ([_VP, ctypes.c_int, ctypes.POINTER(CcgParams), ctypes.POINTER(_VP),
+                      ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(_VP),
+                      _VP, ctypes.POINTER(_VP)], ctypes.c_int),
+    "cm_ccg_step": ([_VP, ctypes.c_int], ctypes.c_int),
+    "cm_ccg_poll": ([_VP, ctypes.POINTER(CcgStats), ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "cm_ccg_finish": ([_VP, ctypes.POINTER(CcgStats)], ctypes.c_int),
+    "cm_ccg_end": ([_VP], ctypes.c_int),
+    "cm_ccg_profile": ([_VP, ctypes.c_int, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
 }
+
+KERNEL_NAMES = ["phase_a", "bisect", "dir_r2c", "col_conv1", "tau_c2r", "phase_b", "force_r2c",
+                "col_conv2", "disp_c2r"]
 
 _lib = None
 _lock = threading.Lock()
 
 
-def load_library(path=LIB_PATH):
-    """Load libcmhip.so and bind every C-ABI symbol (no GPU needed)."""
+def load_library(path=None):
+    """Load libcmhip.so and bind every C-ABI symbol (no GPU needed).
+
+    ``CMHIP_LIB`` may name another build of the same library (A/B timing)."""
     global _lib
+    path = path or os.environ.get("CMHIP_LIB", LIB_PATH)
     with _lock:
         if _lib is not None:
             return _lib

@@ -3,8 +3,8 @@
 // reductions in FP64.
 //
 // The FFT engine transforms one "line" (a row or a column of the computational
-// grid) held in LDS by `tpl` cooperating threads.  Every thread owns E = L/tpl
-// elements; for a radix-R pass it runs E/R butterflies.  A pass loads all of a
+// grid) held in LDS by `tpl` cooperating threads.  A radix-R pass has L/R
+// butterflies dealt round-robin to the threads.  A pass loads all of a
 // thread's butterfly inputs into registers, synchronises, computes and writes
 // the outputs back in place (Stockham index map), then synchronises again, so
 // one LDS buffer of L complex values per line suffices.
@@ -110,6 +110,59 @@ template <typename T, bool INV> struct Dft<T, 8, INV> {
   }
 };
 
+// Radix 16 as 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2; twiddles W16^(n2 k1).
+template <typename T, bool INV>
+__device__ __forceinline__ cplx<T> w16(int m) {
+  const T c = T(0.92387953251128675612818318939679);  // cos(pi/8)
+  const T s = T(0.38268343236508977172845998403040);  // sin(pi/8)
+  const T h = T(0.70710678118654752440084436210485);
+  cplx<T> w;
+  switch (m) {
+    case 1: w = mk(c, -s); break;
+    case 2: w = mk(h, -h); break;
+    case 3: w = mk(s, -c); break;
+    case 4: w = mk(T(0), T(-1)); break;
+    case 6: w = mk(-h, -h); break;
+    default: w = mk(-c, s); break;  // m = 9
+  }
+  return INV ? conj(w) : w;
+}
+
+template <typename T, bool INV> struct Dft<T, 16, INV> {
+  __device__ __forceinline__ static void run(cplx<T>* v) {
+    cplx<T> y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      cplx<T> a[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
+      Dft<T, 4, INV>::run(a);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = a[k1];
+    }
+#pragma unroll
+    for (int n2 = 1; n2 < 4; ++n2)
+#pragma unroll
+      for (int k1 = 1; k1 < 4; ++k1) y[n2][k1] = cmul(y[n2][k1], w16<T, INV>(n2 * k1));
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      cplx<T> b[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+      Dft<T, 4, INV>::run(b);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = b[k2];
+    }
+  }
+};
+
+// LDS keeps complex values at padded positions: one slot is skipped every
+// 128 bytes, so the stride-R Stockham writes of the early passes spread over
+// all banks instead of piling onto a few.
+template <typename T> struct LdsPad {
+  static constexpr int P = sizeof(cplx<T>) == 16 ? 3 : 4;
+};
+template <typename T> __device__ __forceinline__ int lpad(int i) { return i + (i >> LdsPad<T>::P); }
+template <typename T> inline size_t lds_bytes(size_t n) {
+  return (n + (n >> LdsPad<T>::P) + 1) * sizeof(cplx<T>);
+}
+
 // ---------------------------------------------------------------------------
 // FFT plan for one line length, passed by value to kernels.
 struct LinePlan {
@@ -134,7 +187,7 @@ __device__ __forceinline__ void stockham_pass(cplx<T>* lds, int base, int stride
     if (b < nb) {
       const int j = tl + b * tpl;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = lds[base + (j + r * LR) * stride];
+      for (int r = 0; r < R; ++r) v[b][r] = lds[lpad<T>(base + (j + r * LR) * stride)];
     }
   }
   __syncthreads();
@@ -154,7 +207,7 @@ __device__ __forceinline__ void stockham_pass(cplx<T>* lds, int base, int stride
       Dft<T, R, INV>::run(v[b]);
       const int o = (j / Ns) * Ns * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[base + (o + r * Ns) * stride] = v[b][r];
+      for (int r = 0; r < R; ++r) lds[lpad<T>(base + (o + r * Ns) * stride)] = v[b][r];
     }
   }
   __syncthreads();
@@ -175,6 +228,7 @@ __device__ void fft_line(cplx<T>* lds, int base, int stride, const LinePlan& p, 
       case 3: stockham_pass<T, 3, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
       case 4: stockham_pass<T, 4, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
       case 5: stockham_pass<T, 5, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
+      case 16: stockham_pass<T, 16, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
       default: stockham_pass<T, 8, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
     }
     Ns *= R;

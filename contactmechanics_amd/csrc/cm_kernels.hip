@@ -135,10 +135,66 @@ __device__ bool reduce_blocks(double* acc, double* part, unsigned* ctr, int nblo
 }
 
 // ---------------------------------------------------------------------------
+// Per-point functors work on pairs of horizontally adjacent points (col even):
+//   Rec fetch(g, row, col)                 loads only;
+//   make(g, row, col, rec, a, b, acc)      producer: outputs + FFT input pair;
+//   take(g, row, col, rec, a, b, acc)      consumer of an FFT output pair;
+//   emit(g, row, col, rec, acc)            element-wise kernel body.
+// Kernels fetch a batch of kPairs pairs per thread before computing any, so
+// the loads are in flight together instead of waiting behind stores.
+constexpr int kPairs = 4;
+
+template <typename T> struct Vec2;
+template <> struct Vec2<float> { using type = float2; };
+template <> struct Vec2<double> { using type = double2; };
+
+template <typename T>
+struct P2 {
+  T a, b;
+};
+template <typename T>
+__device__ __forceinline__ P2<T> ld2(const T* p) {
+  const typename Vec2<T>::type v = *reinterpret_cast<const typename Vec2<T>::type*>(p);
+  return P2<T>{v.x, v.y};
+}
+template <typename T>
+__device__ __forceinline__ void st2(T* p, T a, T b) {
+  typename Vec2<T>::type v;
+  v.x = a;
+  v.y = b;
+  *reinterpret_cast<typename Vec2<T>::type*>(p) = v;
+}
+__device__ __forceinline__ P2<uint8_t> ldm(const uint8_t* p) {
+  const uint16_t v = *reinterpret_cast<const uint16_t*>(p);
+  return P2<uint8_t>{(uint8_t)(v & 0xff), (uint8_t)(v >> 8)};
+}
+__device__ __forceinline__ void stm(uint8_t* p, uint8_t a, uint8_t b) {
+  *reinterpret_cast<uint16_t*>(p) = (uint16_t)(a | (b << 8));
+}
+
+// Heights and validity of the pair (row, col), (row, col+1).
+struct HPair {
+  double h0, h1;
+  bool c0, c1;  // in the computational region
+};
+__device__ __forceinline__ HPair ldh(const Geom& g, const double* h, const uint8_t* surf, int row, int col) {
+  HPair r;
+  r.c0 = row < g.nx && col < g.ny;
+  r.c1 = row < g.nx && col + 1 < g.ny;
+  const size_t q = (size_t)row * g.ny + col;
+  if (surf) {
+    r.c0 = r.c0 && surf[q];
+    r.c1 = r.c1 && surf[q + 1];
+  }
+  r.h0 = r.c0 ? h[q] : 0.0;
+  r.h1 = r.c1 ? h[q + 1] : 0.0;
+  return r;
+}
+
 // Row pass: real rows -> packed half spectrum (forward, unnormalised).
 template <typename T, int EMAX, class P>
 __global__ void __launch_bounds__(256) k_r2c(Geom g, LinePlan lp, const cplx<T>* __restrict__ tw,
-                                              const cplx<T>* __restrict__ twp, InstArr<T> ia, int ctr_slot) {
+                                             const cplx<T>* __restrict__ twp, InstArr<T> ia, int ctr_slot) {
   extern __shared__ __align__(16) unsigned char smem_r2c[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_r2c);
   const Inst<T>& in = ia.get(blockIdx.y);
@@ -150,11 +206,24 @@ __global__ void __launch_bounds__(256) k_r2c(Geom g, LinePlan lp, const cplx<T>*
   double acc[P::NS > 0 ? P::NS : 1];
 #pragma unroll
   for (int i = 0; i < P::NS; ++i) acc[i] = red_identity(P::op(i));
-  cplx<T>* L = lds + r * N2;
-  for (int n = tl; n < N2; n += lp.tpl) {
-    T a, b;
-    p.load(g, row, 2 * n, a, b, acc);
-    L[n] = mk(a, b);
+  const int rb = r * N2;
+  auto L = [&](int n) -> cplx<T>& { return lds[lpad<T>(rb + n)]; };
+  for (int n0 = tl; n0 < N2; n0 += kPairs * lp.tpl) {
+    typename P::Rec rec[kPairs];
+#pragma unroll
+    for (int k = 0; k < kPairs; ++k) {
+      const int n = n0 + k * lp.tpl;
+      if (n < N2) rec[k] = p.fetch(g, row, 2 * n);
+    }
+#pragma unroll
+    for (int k = 0; k < kPairs; ++k) {
+      const int n = n0 + k * lp.tpl;
+      if (n < N2) {
+        T a, b;
+        p.make(g, row, 2 * n, rec[k], a, b, acc);
+        L(n) = mk(a, b);
+      }
+    }
   }
   __syncthreads();
   fft_line<T, false, EMAX>(lds, r * N2, 1, lp, tl, true, tw);
@@ -162,17 +231,17 @@ __global__ void __launch_bounds__(256) k_r2c(Geom g, LinePlan lp, const cplx<T>*
   // k = 0..N2, packing X[0] (real) and X[N2] (real) into slot 0.
   for (int k = tl; k <= N2 / 2; k += lp.tpl) {
     if (k == 0) {
-      cplx<T> z = L[0];
-      L[0] = mk(z.x + z.y, z.x - z.y);
+      cplx<T> z = L(0);
+      L(0) = mk(z.x + z.y, z.x - z.y);
     } else {
       const int km = N2 - k;
-      cplx<T> zk = L[k], zm = L[km];
+      cplx<T> zk = L(k), zm = L(km);
       cplx<T> E = (zk + conj(zm)) * T(0.5);
       cplx<T> D = zk - conj(zm);
       cplx<T> O = mk(D.y * T(0.5), -D.x * T(0.5));
       cplx<T> B = cmul(twp[k], O);
-      L[k] = E + B;
-      if (km != k) L[km] = conj(E - B);
+      L(k) = E + B;
+      if (km != k) L(km) = conj(E - B);
     }
   }
   __syncthreads();
@@ -181,7 +250,7 @@ __global__ void __launch_bounds__(256) k_r2c(Geom g, LinePlan lp, const cplx<T>*
   const int tot = g.R * N2;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int kk = i % g.Cb, rr = (i / g.Cb) % g.R, cb = i / RC;
-    S[((size_t)cb * g.Mx + blockIdx.x * g.R + rr) * g.Cb + kk] = lds[rr * N2 + cb * g.Cb + kk];
+    S[((size_t)cb * g.Mx + blockIdx.x * g.R + rr) * g.Cb + kk] = lds[lpad<T>(rr * N2 + cb * g.Cb + kk)];
   }
   if constexpr (P::NS > 0) {
     if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
@@ -207,7 +276,7 @@ __global__ void __launch_bounds__(256) k_col(Geom g, LinePlan lp, const cplx<T>*
   const int Mx = g.Mx, Cb = g.Cb;
   const size_t base = (size_t)cb * Mx * Cb;
   const int n = Mx * Cb;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = S[base + i];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) lds[lpad<T>(i)] = S[base + i];
   __syncthreads();
   const int c = threadIdx.x % Cb, tl = threadIdx.x / Cb;
   if (MODE != 2) fft_line<T, false, EMAX>(lds, c, Cb, lp, tl, true, tw);
@@ -215,7 +284,10 @@ __global__ void __launch_bounds__(256) k_col(Geom g, LinePlan lp, const cplx<T>*
     const int ky = cb * Cb + c;
     if (ky != 0) {
       const T* Gk = G + (size_t)ky * Mx;
-      for (int x = tl; x < Mx; x += lp.tpl) lds[x * Cb + c] = lds[x * Cb + c] * Gk[x];
+      for (int x = tl; x < Mx; x += lp.tpl) {
+        cplx<T>& e = lds[lpad<T>(x * Cb + c)];
+        e = e * Gk[x];
+      }
     } else {
       // packed DC/Nyquist column: Q = A + iB with A, B spectra of real
       // columns; A *= G(kx, 0), B *= G(kx, My/2).
@@ -223,25 +295,25 @@ __global__ void __launch_bounds__(256) k_col(Geom g, LinePlan lp, const cplx<T>*
       const T* GN = G + (size_t)g.N2 * Mx;
       for (int x = tl; x <= Mx / 2; x += lp.tpl) {
         const int xm = (Mx - x) % Mx;
-        cplx<T> q = lds[x * Cb], qm = lds[xm * Cb];
+        cplx<T> q = lds[lpad<T>(x * Cb)], qm = lds[lpad<T>(xm * Cb)];
         const T a0 = G0[x], aN = GN[x], b0 = G0[xm], bN = GN[xm];
         cplx<T> nq = q * (T(0.5) * (a0 + aN)) + conj(qm) * (T(0.5) * (a0 - aN));
         cplx<T> nqm = qm * (T(0.5) * (b0 + bN)) + conj(q) * (T(0.5) * (b0 - bN));
-        lds[x * Cb] = nq;
-        if (xm != x) lds[xm * Cb] = nqm;
+        lds[lpad<T>(x * Cb)] = nq;
+        if (xm != x) lds[lpad<T>(xm * Cb)] = nqm;
       }
     }
     __syncthreads();
   }
   if (MODE != 1) fft_line<T, true, EMAX>(lds, c, Cb, lp, tl, true, tw);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) S[base + i] = lds[i];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) S[base + i] = lds[lpad<T>(i)];
 }
 
 // Row pass: packed half spectrum -> real rows (inverse, unnormalised; the
 // round trip r2c -> c2r scales by Mx*My, which the kernels G/K absorb).
 template <typename T, int EMAX, class P>
 __global__ void __launch_bounds__(256) k_c2r(Geom g, LinePlan lp, const cplx<T>* __restrict__ tw,
-                                              const cplx<T>* __restrict__ twp, InstArr<T> ia, int ctr_slot) {
+                                             const cplx<T>* __restrict__ twp, InstArr<T> ia, int ctr_slot) {
   extern __shared__ __align__(16) unsigned char smem_c2r[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_c2r);
   const Inst<T>& in = ia.get(blockIdx.y);
@@ -253,23 +325,24 @@ __global__ void __launch_bounds__(256) k_c2r(Geom g, LinePlan lp, const cplx<T>*
   const int tot = g.R * N2;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int kk = i % g.Cb, rr = (i / g.Cb) % g.R, cb = i / RC;
-    lds[rr * N2 + cb * g.Cb + kk] = S[((size_t)cb * g.Mx + blockIdx.x * g.R + rr) * g.Cb + kk];
+    lds[lpad<T>(rr * N2 + cb * g.Cb + kk)] = S[((size_t)cb * g.Mx + blockIdx.x * g.R + rr) * g.Cb + kk];
   }
   __syncthreads();
   const int r = threadIdx.x / lp.tpl, tl = threadIdx.x - r * lp.tpl;
   const int row = blockIdx.x * g.R + r;
-  cplx<T>* L = lds + r * N2;
+  const int rb = r * N2;
+  auto L = [&](int n) -> cplx<T>& { return lds[lpad<T>(rb + n)]; };
   for (int k = tl; k <= N2 / 2; k += lp.tpl) {
     if (k == 0) {
-      cplx<T> z = L[0];  // (X[0], X[N2])
-      L[0] = mk(z.x + z.y, z.x - z.y);
+      cplx<T> z = L(0);  // (X[0], X[N2])
+      L(0) = mk(z.x + z.y, z.x - z.y);
     } else {
       const int km = N2 - k;
-      cplx<T> xk = L[k], xm = L[km];
+      cplx<T> xk = L(k), xm = L(km);
       cplx<T> E = xk + conj(xm);
       cplx<T> O = cmulc(xk - conj(xm), twp[k]);
-      L[k] = mk(E.x - O.y, E.y + O.x);            // E + iO
-      if (km != k) L[km] = mk(E.x + O.y, O.x - E.y);  // conj(E) + i conj(O)
+      L(k) = mk(E.x - O.y, E.y + O.x);            // E + iO
+      if (km != k) L(km) = mk(E.x + O.y, O.x - E.y);  // conj(E) + i conj(O)
     }
   }
   __syncthreads();
@@ -277,9 +350,21 @@ __global__ void __launch_bounds__(256) k_c2r(Geom g, LinePlan lp, const cplx<T>*
   double acc[P::NS > 0 ? P::NS : 1];
 #pragma unroll
   for (int i = 0; i < P::NS; ++i) acc[i] = red_identity(P::op(i));
-  for (int n = tl; n < N2; n += lp.tpl) {
-    cplx<T> z = L[n];
-    p.store(g, row, 2 * n, z.x, z.y, acc);
+  for (int n0 = tl; n0 < N2; n0 += kPairs * lp.tpl) {
+    typename P::Rec rec[kPairs];
+#pragma unroll
+    for (int k = 0; k < kPairs; ++k) {
+      const int n = n0 + k * lp.tpl;
+      if (n < N2) rec[k] = p.fetch(g, row, 2 * n);
+    }
+#pragma unroll
+    for (int k = 0; k < kPairs; ++k) {
+      const int n = n0 + k * lp.tpl;
+      if (n < N2) {
+        const cplx<T> z = L(n);
+        p.take(g, row, 2 * n, rec[k], z.x, z.y, acc);
+      }
+    }
   }
   if constexpr (P::NS > 0) {
     if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
@@ -287,7 +372,8 @@ __global__ void __launch_bounds__(256) k_c2r(Geom g, LinePlan lp, const cplx<T>*
   }
 }
 
-// Grid-stride element-wise kernel with fused reductions.
+// Element-wise kernel with fused reductions: blocks stride over rows, threads
+// over point pairs of a row (no index division, coalesced pair loads).
 template <typename T, class P>
 __global__ void __launch_bounds__(256) k_elem(Geom g, InstArr<T> ia, int ctr_slot) {
   const Inst<T>& in = ia.get(blockIdx.y);
@@ -296,10 +382,21 @@ __global__ void __launch_bounds__(256) k_elem(Geom g, InstArr<T> ia, int ctr_slo
   double acc[P::NS > 0 ? P::NS : 1];
 #pragma unroll
   for (int i = 0; i < P::NS; ++i) acc[i] = red_identity(P::op(i));
-  const size_t M = (size_t)g.Mx * g.My;
-  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < M; idx += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(idx / g.My), j = (int)(idx - (size_t)i * g.My);
-    p.point(g, idx, i, j, acc);
+  const int npair = g.My / 2;
+  for (int row = blockIdx.x; row < g.Mx; row += gridDim.x) {
+    for (int n0 = threadIdx.x; n0 < npair; n0 += kPairs * blockDim.x) {
+      typename P::Rec rec[kPairs];
+#pragma unroll
+      for (int k = 0; k < kPairs; ++k) {
+        const int n = n0 + k * blockDim.x;
+        if (n < npair) rec[k] = p.fetch(g, row, 2 * n);
+      }
+#pragma unroll
+      for (int k = 0; k < kPairs; ++k) {
+        const int n = n0 + k * blockDim.x;
+        if (n < npair) p.emit(g, row, 2 * n, rec[k], acc);
+      }
+    }
   }
   if constexpr (P::NS > 0) {
     if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
@@ -314,13 +411,16 @@ template <typename T>
 struct ProdCopy {  // input row from src
   static constexpr int NS = 0;
   __device__ static constexpr int op(int) { return RSUM; }
+  struct Rec {
+    P2<T> v;
+  };
   const Inst<T>& in;
   __device__ ProdCopy(const Inst<T>& i) : in(i) {}
   __device__ bool skip() const { return false; }
-  __device__ void load(const Geom& g, int row, int col, T& a, T& b, double*) const {
-    const size_t p = (size_t)row * g.My + col;
-    a = in.src[p];
-    b = in.src[p + 1];
+  __device__ Rec fetch(const Geom& g, int row, int col) const { return Rec{ld2(in.src + (size_t)row * g.My + col)}; }
+  __device__ void make(const Geom&, int, int, const Rec& r, T& a, T& b, double*) const {
+    a = r.v.a;
+    b = r.v.b;
   }
   __device__ void finalize(double*) const {}
 };
@@ -329,13 +429,13 @@ template <typename T>
 struct ConsCopy {  // output row to dst
   static constexpr int NS = 0;
   __device__ static constexpr int op(int) { return RSUM; }
+  struct Rec {};
   const Inst<T>& in;
   __device__ ConsCopy(const Inst<T>& i) : in(i) {}
   __device__ bool skip() const { return false; }
-  __device__ void store(const Geom& g, int row, int col, T a, T b, double*) const {
-    const size_t p = (size_t)row * g.My + col;
-    in.dst[p] = a;
-    in.dst[p + 1] = b;
+  __device__ Rec fetch(const Geom&, int, int) const { return Rec{}; }
+  __device__ void take(const Geom& g, int row, int col, const Rec&, T a, T b, double*) const {
+    st2(in.dst + (size_t)row * g.My + col, a, b);
   }
   __device__ void finalize(double*) const {}
 };
@@ -348,17 +448,32 @@ template <typename T>
 struct ElemInit {
   static constexpr int NS = 2;  // max_h, n_comp
   __device__ static constexpr int op(int i) { return i == 0 ? RMAX : RSUM; }
+  struct Rec {
+    P2<T> u;
+    HPair h;
+  };
   const Inst<T>& in;
   __device__ ElemInit(const Inst<T>& i) : in(i) {}
   __device__ bool skip() const { return false; }
-  __device__ void point(const Geom& g, size_t p, int i, int j, double* acc) const {
-    if (!is_comp(g, in.surf, i, j)) return;
-    const double hv = in.h[i * g.ny + j];
-    const double lim = hv + in.sc->offset;
-    const double uv = (double)in.u[p];
-    if (uv < lim) in.u[p] = (T)lim;
-    acc[0] = red_apply(RMAX, acc[0], hv);
-    acc[1] += 1.0;
+  __device__ Rec fetch(const Geom& g, int row, int col) const {
+    return Rec{ld2(in.u + (size_t)row * g.My + col), ldh(g, in.h, in.surf, row, col)};
+  }
+  __device__ void emit(const Geom& g, int row, int col, const Rec& r, double* acc) const {
+    const double off = in.sc->offset;
+    T u0 = r.u.a, u1 = r.u.b;
+    if (r.h.c0) {
+      const double lim = r.h.h0 + off;
+      if ((double)u0 < lim) u0 = (T)lim;
+      acc[0] = red_apply(RMAX, acc[0], r.h.h0);
+      acc[1] += 1.0;
+    }
+    if (r.h.c1) {
+      const double lim = r.h.h1 + off;
+      if ((double)u1 < lim) u1 = (T)lim;
+      acc[0] = red_apply(RMAX, acc[0], r.h.h1);
+      acc[1] += 1.0;
+    }
+    if (r.h.c0 || r.h.c1) st2(in.u + (size_t)row * g.My + col, u0, u1);
   }
   // the host stores the full grid size M in n_pad; subtract the comp count
   __device__ void finalize(double* acc) const {
@@ -373,13 +488,15 @@ template <typename T>
 struct ConsInitForce {
   static constexpr int NS = 0;
   __device__ static constexpr int op(int) { return RSUM; }
+  struct Rec {
+    HPair h;
+  };
   const Inst<T>& in;
   __device__ ConsInitForce(const Inst<T>& i) : in(i) {}
   __device__ bool skip() const { return false; }
-  __device__ void store(const Geom& g, int row, int col, T a, T b, double*) const {
-    const size_t p = (size_t)row * g.My + col;
-    in.f[p] = is_comp(g, in.surf, row, col) ? a : T(0);
-    in.f[p + 1] = is_comp(g, in.surf, row, col + 1) ? b : T(0);
+  __device__ Rec fetch(const Geom& g, int row, int col) const { return Rec{ldh(g, in.h, in.surf, row, col)}; }
+  __device__ void take(const Geom& g, int row, int col, const Rec& r, T a, T b, double*) const {
+    st2(in.f + (size_t)row * g.My + col, r.h.c0 ? a : T(0), r.h.c1 ? b : T(0));
   }
   __device__ void finalize(double*) const {}
 };
@@ -389,11 +506,17 @@ template <typename T>
 struct ElemZeroPad {
   static constexpr int NS = 0;
   __device__ static constexpr int op(int) { return RSUM; }
+  struct Rec {
+    HPair h;
+  };
   const Inst<T>& in;
   __device__ ElemZeroPad(const Inst<T>& i) : in(i) {}
   __device__ bool skip() const { return false; }
-  __device__ void point(const Geom& g, size_t p, int i, int j, double*) const {
-    if (!is_comp(g, in.surf, i, j)) in.f[p] = T(0);
+  __device__ Rec fetch(const Geom& g, int row, int col) const { return Rec{ldh(g, in.h, in.surf, row, col)}; }
+  __device__ void emit(const Geom& g, int row, int col, const Rec& r, double*) const {
+    T* fp = in.f + (size_t)row * g.My + col;
+    if (!r.h.c0) fp[0] = T(0);
+    if (!r.h.c1) fp[1] = T(0);
   }
   __device__ void finalize(double*) const {}
 };
@@ -407,25 +530,37 @@ template <typename T>
 struct ElemPhaseA {
   static constexpr int NS = 6;  // A_contact, A_cg, Sd, Sdd, gmax, gmin
   __device__ static constexpr int op(int i) { return i == 4 ? RMAX : (i == 5 ? RMIN : RSUM); }
+  struct Rec {
+    P2<T> f, u;
+    HPair h;
+  };
   const Inst<T>& in;
   __device__ ElemPhaseA(const Inst<T>& i) : in(i) {}
   __device__ bool skip() const { return in.sc->done != 0; }
-  __device__ void point(const Geom& g, size_t p, int i, int j, double* acc) const {
-    const Scal* s = in.sc;
-    const double fv = (double)in.f[p];
+  __device__ Rec fetch(const Geom& g, int row, int col) const {
+    const size_t p = (size_t)row * g.My + col;
+    return Rec{ld2(in.f + p), ld2(in.u + p), ldh(g, in.h, in.surf, row, col)};
+  }
+  __device__ bool one(const Scal* s, double fv, double uv, double hv, bool comp, double* acc) const {
     const bool c0 = fv < 0.0;
     const bool c = c0 && (!s->has_H || fv > -s->H);
     acc[0] += c0 ? 1.0 : 0.0;
     acc[1] += c ? 1.0 : 0.0;
-    in.cm[p] = c ? 1 : 0;
-    if (is_comp(g, in.surf, i, j)) {
-      const double g0 = (double)in.u[p] - in.h[i * g.ny + j];
+    if (comp) {
+      const double g0 = uv - hv;
       const double d = g0 - s->shift;
-      if (c) acc[2] += d;  // sum over the contact set only (:269)
-      acc[3] += (c ? 1.0 : 0.0) * d * d;  // c * g * g: a NaN gap poisons G as in :273
+      if (c) acc[2] += d;                  // sum over the contact set only (:269)
+      acc[3] += (c ? 1.0 : 0.0) * d * d;   // c * g * g: a NaN gap poisons G as in :273
       acc[4] = red_apply(RMAX, acc[4], g0);
       acc[5] = red_apply(RMIN, acc[5], g0);
     }
+    return c;
+  }
+  __device__ void emit(const Geom& g, int row, int col, const Rec& r, double* acc) const {
+    const Scal* s = in.sc;
+    const bool c0 = one(s, (double)r.f.a, (double)r.u.a, r.h.h0, r.h.c0, acc);
+    const bool c1 = one(s, (double)r.f.b, (double)r.u.b, r.h.h1, r.h.c1, acc);
+    stm(in.cm + (size_t)row * g.My + col, c0 ? 1 : 0, c1 ? 1 : 0);
   }
   __device__ void finalize(double* a) const {
     Scal* s = in.sc;
@@ -524,25 +659,32 @@ template <typename T>
 struct ProdDir {
   static constexpr int NS = 1;
   __device__ static constexpr int op(int) { return RSUM; }
+  struct Rec {
+    P2<T> u, t;
+    P2<uint8_t> cm;
+    HPair h;
+  };
   const Inst<T>& in;
   __device__ ProdDir(const Inst<T>& i) : in(i) {}
   __device__ bool skip() const { return in.sc->done || in.sc->mode == MODE_MIX; }
-  __device__ T one(const Geom& g, int row, int col, double* acc) const {
+  __device__ Rec fetch(const Geom& g, int row, int col) const {
     const size_t p = (size_t)row * g.My + col;
-    if (!is_comp(g, in.surf, row, col)) return in.t[p];
-    const Scal* s = in.sc;
-    const bool c = (in.cm[p] & 1) != 0;
-    const double gap = ((double)in.u[p] - in.h[row * g.ny + col]) - s->offset;
+    return Rec{ld2(in.u + p), ld2(in.t + p), ldm(in.cm + p), ldh(g, in.h, in.surf, row, col)};
+  }
+  __device__ T one(const Scal* s, bool comp, bool c, double uv, double hv, T told, double* acc) const {
+    if (!comp) return told;  // t is only ever set on comp (zero elsewhere)
+    const double gap = (uv - hv) - s->offset;
     double tn = 0.0;
-    if (c) tn = s->use_beta ? gap + s->beta * (double)in.t[p] : gap;
+    if (c) tn = s->use_beta ? gap + s->beta * (double)told : gap;
     const T tt = (T)tn;
-    in.t[p] = tt;
     if (c) acc[0] += gap * (double)tt;
     return tt;
   }
-  __device__ void load(const Geom& g, int row, int col, T& a, T& b, double* acc) const {
-    a = one(g, row, col, acc);
-    b = one(g, row, col + 1, acc);
+  __device__ void make(const Geom& g, int row, int col, const Rec& r, T& a, T& b, double* acc) const {
+    const Scal* s = in.sc;
+    a = one(s, r.h.c0, r.cm.a & 1, (double)r.u.a, r.h.h0, r.t.a, acc);
+    b = one(s, r.h.c1, r.cm.b & 1, (double)r.u.b, r.h.h1, r.t.b, acc);
+    st2(in.t + (size_t)row * g.My + col, a, b);
   }
   __device__ void finalize(double* a) const { in.sc->Scgt = a[0]; }
 };
@@ -553,13 +695,20 @@ template <typename T>
 struct ConsTau {
   static constexpr int NS = 1;
   __device__ static constexpr int op(int) { return RSUM; }
+  struct Rec {
+    P2<T> t;
+    P2<uint8_t> cm;
+  };
   const Inst<T>& in;
   __device__ ConsTau(const Inst<T>& i) : in(i) {}
   __device__ bool skip() const { return in.sc->done || in.sc->mode == MODE_MIX; }
-  __device__ void store(const Geom& g, int row, int col, T a, T b, double* acc) const {
+  __device__ Rec fetch(const Geom& g, int row, int col) const {
     const size_t p = (size_t)row * g.My + col;
-    if (in.cm[p] & 1) acc[0] += (double)a * (double)in.t[p];
-    if (in.cm[p + 1] & 1) acc[0] += (double)b * (double)in.t[p + 1];
+    return Rec{ld2(in.t + p), ldm(in.cm + p)};
+  }
+  __device__ void take(const Geom&, int, int, const Rec& r, T a, T b, double* acc) const {
+    if (r.cm.a & 1) acc[0] += (double)a * (double)r.t.a;
+    if (r.cm.b & 1) acc[0] += (double)b * (double)r.t.b;
   }
   __device__ void finalize(double* a) const {
     Scal* s = in.sc;
@@ -577,13 +726,10 @@ struct ConsTau {
 
 // Per-pixel force after the PK or mixing update, before projection
 // (:296 and :322-323).  Shared by phase B and the second transform.
-template <typename T>
-struct ForceUpdate {
-  __device__ static __forceinline__ double f1(const Scal* s, double fv, double tv, bool c) {
-    if (s->mode == MODE_PK) return fv + (c ? s->tau * tv : 0.0);
-    return (1.0 - s->mix) * fv - (s->mix * s->H) * (c ? 1.0 : 0.0);
-  }
-};
+__device__ __forceinline__ double force_update(const Scal* s, double fv, double tv, bool c) {
+  if (s->mode == MODE_PK) return fv + (c ? s->tau * tv : 0.0);
+  return (1.0 - s->mix) * fv - (s->mix * s->H) * (c ? 1.0 : 0.0);
+}
 
 // Phase B: projection statistics (:332-389).
 template <typename T>
@@ -592,20 +738,25 @@ struct ElemPhaseB {
   __device__ static constexpr int op(int i) {
     return (i == 1 || i == 2) ? RMAX : (i == 4 ? RMIN : RSUM);
   }
+  struct Rec {
+    P2<T> f, t, u;
+    P2<uint8_t> cm;
+    HPair h;
+  };
   const Inst<T>& in;
   __device__ ElemPhaseB(const Inst<T>& i) : in(i) {}
   __device__ bool skip() const { return in.sc->done != 0; }
-  __device__ void point(const Geom& g, size_t p, int i, int j, double* acc) const {
-    const Scal* s = in.sc;
-    const bool comp = is_comp(g, in.surf, i, j);
+  __device__ Rec fetch(const Geom& g, int row, int col) const {
+    const size_t p = (size_t)row * g.My + col;
+    return Rec{ld2(in.f + p), ld2(in.t + p), ld2(in.u + p), ldm(in.cm + p), ldh(g, in.h, in.surf, row, col)};
+  }
+  __device__ uint8_t one(const Scal* s, bool comp, uint8_t cmv, double fv, double tv, double uv, double hv,
+                         double* acc) const {
     double gap = 0.0;
-    if (comp) gap = ((double)in.u[p] - in.h[i * g.ny + j]) - s->offset;
-    bool c = (in.cm[p] & 1) != 0;
-    if (s->mode == MODE_MIX && comp) {  // c[comp] = g < 0 (:322)
-      c = gap < 0.0;
-      in.cm[p] = c ? 1 : 0;
-    }
-    const double f1 = ForceUpdate<T>::f1(s, (double)in.f[p], (double)in.t[p], c);
+    if (comp) gap = (uv - hv) - s->offset;
+    bool c = (cmv & 1) != 0;
+    if (s->mode == MODE_MIX && comp) c = gap < 0.0;  // c[comp] = g < 0 (:322)
+    const double f1 = force_update(s, fv, tv, c);
     const bool tensile = f1 >= 0.0;
     const bool flowing = s->has_H && f1 <= -s->H;
     if (tensile) {
@@ -622,6 +773,13 @@ struct ElemPhaseB {
       const bool nc = (tensile && gap < 0.0) || (flowing && gap > 0.0);
       acc[6] += nc ? 1.0 : 0.0;
     }
+    return c ? 1 : 0;
+  }
+  __device__ void emit(const Geom& g, int row, int col, const Rec& r, double* acc) const {
+    const Scal* s = in.sc;
+    const uint8_t c0 = one(s, r.h.c0, r.cm.a, (double)r.f.a, (double)r.t.a, (double)r.u.a, r.h.h0, acc);
+    const uint8_t c1 = one(s, r.h.c1, r.cm.b, (double)r.f.b, (double)r.t.b, (double)r.u.b, r.h.h1, acc);
+    if (s->mode == MODE_MIX) stm(in.cm + (size_t)row * g.My + col, c0, c1);
   }
   __device__ void finalize(double* a) const {
     Scal* s = in.sc;
@@ -662,40 +820,51 @@ template <typename T>
 struct ProdForce {
   static constexpr int NS = 1;  // sum f over comp
   __device__ static constexpr int op(int) { return RSUM; }
+  struct Rec {
+    P2<T> f, t, u;
+    P2<uint8_t> cm;
+    HPair h;
+  };
   const Inst<T>& in;
   __device__ ProdForce(const Inst<T>& i) : in(i) {}
   __device__ bool skip() const { return in.sc->done != 0; }
-  __device__ T one(const Geom& g, int row, int col, double* acc) const {
-    const Scal* s = in.sc;
+  __device__ Rec fetch(const Geom& g, int row, int col) const {
     const size_t p = (size_t)row * g.My + col;
-    const bool comp = is_comp(g, in.surf, row, col);
+    return Rec{ld2(in.f + p), ld2(in.t + p), ld2(in.u + p), ldm(in.cm + p), ldh(g, in.h, in.surf, row, col)};
+  }
+  __device__ T one(const Scal* s, bool comp, uint8_t& cmv, double fv, double tv, double uv, double hv,
+                   double* acc) const {
     double gap = 0.0;
-    if (comp) gap = ((double)in.u[p] - in.h[row * g.ny + col]) - s->offset;
-    const bool c = (in.cm[p] & 1) != 0;
-    const double f1 = ForceUpdate<T>::f1(s, (double)in.f[p], (double)in.t[p], c);
+    if (comp) gap = (uv - hv) - s->offset;
+    const bool c = (cmv & 1) != 0;
+    const double f1 = force_update(s, fv, tv, c);
     const bool tensile = f1 >= 0.0;
     const bool flowing = s->has_H && f1 <= -s->H;
-    double fv = tensile ? 0.0 : f1;                       // :359
+    double fo = tensile ? 0.0 : f1;                       // :359
     if (s->load_ctl) {
       if (s->uniform)
-        fv = comp ? -s->F / s->nb_surface_pts : 0.0;      // :367-368
+        fo = comp ? -s->F / s->nb_surface_pts : 0.0;      // :367-368
       else
-        fv = fv * s->scale;                               // :365
+        fo = fo * s->scale;                               // :365
     }
-    if (flowing) fv = -s->H;                              // :372-373
+    if (flowing) fo = -s->H;                              // :372-373
     if (comp) {
       const bool nc = (tensile && gap < 0.0) || (flowing && gap > 0.0);
-      if (s->do_nc && nc) fv = fv + s->tau * gap;         // :384
-      in.cm[p] = (uint8_t)((c ? 1 : 0) | (gap < 0.0 ? 2 : 0));
+      if (s->do_nc && nc) fo = fo + s->tau * gap;         // :384
+      cmv = (uint8_t)((c ? 1 : 0) | (gap < 0.0 ? 2 : 0));
     }
-    const T ft = (T)fv;
-    in.f[p] = ft;
+    const T ft = (T)fo;
     if (comp) acc[0] += (double)ft;
     return ft;
   }
-  __device__ void load(const Geom& g, int row, int col, T& a, T& b, double* acc) const {
-    a = one(g, row, col, acc);
-    b = one(g, row, col + 1, acc);
+  __device__ void make(const Geom& g, int row, int col, const Rec& r, T& a, T& b, double* acc) const {
+    const Scal* s = in.sc;
+    uint8_t m0 = r.cm.a, m1 = r.cm.b;
+    a = one(s, r.h.c0, m0, (double)r.f.a, (double)r.t.a, (double)r.u.a, r.h.h0, acc);
+    b = one(s, r.h.c1, m1, (double)r.f.b, (double)r.t.b, (double)r.u.b, r.h.h1, acc);
+    const size_t p = (size_t)row * g.My + col;
+    st2(in.f + p, a, b);
+    if (r.h.c0 || r.h.c1) stm(in.cm + p, m0, m1);
   }
   __device__ void finalize(double* a) const { in.sc->total_force = -a[0]; }
 };
@@ -706,23 +875,31 @@ template <typename T>
 struct ConsDisp {
   static constexpr int NS = 2;  // maxdu, max_pen
   __device__ static constexpr int op(int) { return RMAX; }
+  struct Rec {
+    P2<T> u;
+    P2<uint8_t> cm;
+    HPair h;
+  };
   const Inst<T>& in;
   __device__ ConsDisp(const Inst<T>& i) : in(i) {}
   __device__ bool skip() const { return in.sc->done != 0; }
-  __device__ void one(const Geom& g, int row, int col, T v, double* acc) const {
+  __device__ Rec fetch(const Geom& g, int row, int col) const {
     const size_t p = (size_t)row * g.My + col;
+    return Rec{ld2(in.u + p), ldm(in.cm + p), ldh(g, in.h, in.surf, row, col)};
+  }
+  __device__ void one(double off, bool comp, uint8_t cmv, double uo, double hv, T v, double* acc) const {
     const double un = (double)v;
-    acc[0] = red_apply(RMAX, acc[0], fabs(un - (double)in.u[p]));
-    in.u[p] = v;
-    if (is_comp(g, in.surf, row, col)) {
-      const bool c = (in.cm[p] & 1) != 0;
-      const double pen = c ? (in.h[row * g.ny + col] + in.sc->offset - un) : 0.0;
+    acc[0] = red_apply(RMAX, acc[0], fabs(un - uo));
+    if (comp) {
+      const double pen = (cmv & 1) ? (hv + off - un) : 0.0;
       acc[1] = red_apply(RMAX, acc[1], pen);
     }
   }
-  __device__ void store(const Geom& g, int row, int col, T a, T b, double* acc) const {
-    one(g, row, col, a, acc);
-    one(g, row, col + 1, b, acc);
+  __device__ void take(const Geom& g, int row, int col, const Rec& r, T a, T b, double* acc) const {
+    const double off = in.sc->offset;
+    one(off, r.h.c0, r.cm.a, (double)r.u.a, r.h.h0, a, acc);
+    one(off, r.h.c1, r.cm.b, (double)r.u.b, r.h.h1, b, acc);
+    st2(in.u + (size_t)row * g.My + col, a, b);
   }
   __device__ void finalize(double* a) const {
     Scal* s = in.sc;
@@ -773,11 +950,19 @@ template <typename T>
 struct ElemEnergy {
   static constexpr int NS = 1;
   __device__ static constexpr int op(int) { return RSUM; }
+  struct Rec {
+    P2<T> f, u;
+  };
   const Inst<T>& in;
   __device__ ElemEnergy(const Inst<T>& i) : in(i) {}
   __device__ bool skip() const { return false; }
-  __device__ void point(const Geom& g, size_t p, int i, int j, double* acc) const {
-    if (i < g.nx && j < g.ny) acc[0] += (double)in.f[p] * (double)in.u[p];
+  __device__ Rec fetch(const Geom& g, int row, int col) const {
+    const size_t p = (size_t)row * g.My + col;
+    return Rec{ld2(in.f + p), ld2(in.u + p)};
+  }
+  __device__ void emit(const Geom& g, int row, int col, const Rec& r, double* acc) const {
+    if (row < g.nx && col < g.ny) acc[0] += (double)r.f.a * (double)r.u.a;
+    if (row < g.nx && col + 1 < g.ny) acc[0] += (double)r.f.b * (double)r.u.b;
   }
   __device__ void finalize(double* a) const { in.sc->fun = -a[0] / 2; }
 };
@@ -885,7 +1070,7 @@ bool make_line_plan(int L, int max_tpl, int emax, LinePlan& best) {
   int rem = L, np = 0;
   while (rem > 1) {
     int pick = 0;
-    for (int R : {8, 4, 5, 3, 2})
+    for (int R : {16, 8, 4, 5, 3, 2})
       if (rem % R == 0) {
         pick = R;
         break;
@@ -969,10 +1154,6 @@ struct cm_plan {
   void* S = nullptr;     // spectrum work buffer for plain operator calls
   size_t bytes_owned = 0;
   int nblk_elem = 1;
-  // persistent solve workspace (grown on demand)
-  size_t ws_B = 0;
-  void* ws = nullptr;
-  size_t ws_bytes = 0;
 };
 
 namespace {
@@ -1001,15 +1182,15 @@ int exec_setup(cm_ctx* ctx, FftExec& ex, int nx, int ny, int Mx, int My, size_t&
   while (Cb * 2 <= cb_target && g.N2 % (Cb * 2) == 0 && (size_t)Mx * Cb * 2 * cs <= 65536 &&
          ex.col.tpl * Cb * 2 <= kMaxBlock)
     Cb *= 2;
-  if ((size_t)Mx * Cb * cs > 160 * 1024) return fail(ctx, CM_E_UNSUPPORTED, "column length exceeds LDS capacity");
+  if (lds_bytes<T>((size_t)Mx * Cb) > 150 * 1024) return fail(ctx, CM_E_UNSUPPORTED, "column length exceeds LDS capacity");
   g.Cb = Cb;
   int R = 1;
   while (Mx % (R * 2) == 0 && ex.row.tpl * R * 2 <= kMaxBlock && (size_t)R * 2 * g.N2 * cs <= 65536 &&
          (ex.row.tpl * R < 128 || (size_t)R * Cb * cs < 128))
     R *= 2;
   g.R = R;
-  ex.lds_row = (size_t)R * g.N2 * cs;
-  ex.lds_col = (size_t)Mx * Cb * cs;
+  ex.lds_row = lds_bytes<T>((size_t)R * g.N2);
+  ex.lds_col = lds_bytes<T>((size_t)Mx * Cb);
   ex.emax_row = ex.row.E <= 16 ? 16 : 32;
   ex.emax_col = ex.col.E <= 16 ? 16 : 32;
   auto twr = host_twiddles<T>(g.N2, g.N2, g.N2);
@@ -1093,8 +1274,7 @@ int plan_free(cm_plan* p) {
   dev_free(ctx, p->G, 0);
   dev_free(ctx, p->K, 0);
   dev_free(ctx, p->S, 0);
-  dev_free(ctx, p->ws, 0);
-  ctx->used -= std::min(ctx->used, p->bytes_owned + p->ws_bytes);
+  ctx->used -= std::min(ctx->used, p->bytes_owned);
   delete p;
   return 0;
 }
@@ -1248,51 +1428,29 @@ struct WsLayout {
   size_t st_t, st_S, st_sc, st_part, st_ctr;  // per-instance strides
 };
 
-template <typename T>
+size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
+
 WsLayout ws_layout(const cm_plan* P, int B) {
+  const size_t w = P->prec == CM_FP64 ? 8 : 4;
   const size_t M = (size_t)P->Mx * P->My;
   const size_t nparts = std::max<size_t>((size_t)P->nblk_elem, (size_t)(P->Mx / P->ex.g.R)) * 8;
-  auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
   WsLayout L{};
-  L.st_t = al(M * sizeof(T));
-  L.st_S = al((M / 2) * sizeof(cplx<T>));
-  L.st_sc = al(sizeof(Scal));
-  L.st_part = al(nparts * sizeof(double));
-  L.st_ctr = al(8 * sizeof(unsigned));
+  L.st_t = al256(M * w);
+  L.st_S = al256((M / 2) * 2 * w);
+  L.st_sc = al256(sizeof(Scal));
+  L.st_part = al256(nparts * sizeof(double));
+  L.st_ctr = al256(8 * sizeof(unsigned));
+  const size_t inst_bytes = al256(std::max(sizeof(Inst<double>), sizeof(Inst<float>)) * B);
   size_t o = 0;
   L.t = o; o += L.st_t * B;
   L.S = o; o += L.st_S * B;
   L.sc = o; o += L.st_sc * B;
   L.part = o; o += L.st_part * B;
   L.ctr = o; o += L.st_ctr * B;
-  L.inst = o; o += al(sizeof(Inst<T>) * B);
-  L.inst0 = o; o += al(sizeof(Inst<T>) * B);
+  L.inst = o; o += inst_bytes;
+  L.inst0 = o; o += inst_bytes;
   L.total = o;
   return L;
-}
-
-// Workspace for B instances: conjugate directions, spectra, scalar state,
-// partial reductions, tickets and the device instance tables.
-template <typename T>
-int ensure_ws(cm_plan* P, int B, WsLayout& L) {
-  if (P->ws && (size_t)B <= P->ws_B) {
-    L = ws_layout<T>(P, (int)P->ws_B);
-    return 0;
-  }
-  cm_ctx* ctx = P->ctx;
-  if (P->ws) {
-    (void)hipStreamSynchronize(ctx->stream);
-    dev_free(ctx, P->ws, P->ws_bytes);
-    P->ws = nullptr;
-    P->ws_bytes = 0;
-    P->ws_B = 0;
-  }
-  L = ws_layout<T>(P, B);
-  int rc = dev_alloc(ctx, &P->ws, L.total);
-  if (rc) return rc;
-  P->ws_bytes = L.total;
-  P->ws_B = B;
-  return 0;
 }
 
 void fill_stats(const Scal& s, cm_ccg_stats* o) {
@@ -1316,39 +1474,86 @@ void fill_stats(const Scal& s, cm_ccg_stats* o) {
   o->nc_count = s.nc_count;
 }
 
-// Pinned host staging that lives until the solve has synchronised.
-template <typename X>
-struct Pinned {
-  X* p = nullptr;
-  explicit Pinned(size_t n) {
-    if (hipHostMalloc((void**)&p, sizeof(X) * (n ? n : 1), 0) != hipSuccess) p = nullptr;
-  }
-  ~Pinned() {
-    if (p) (void)hipHostFree(p);
-  }
+}  // namespace
+
+// A solve in flight: B instances of one plan, their workspace (conjugate
+// directions, spectra, scalar state, partial reductions) resident on the
+// device between calls, and the captured iteration graphs.
+struct cm_session {
+  cm_plan* P = nullptr;
+  int B = 0;
+  hipStream_t st = nullptr;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  WsLayout L{};
+  bool any_bisect = false;
+  int use_graph = 0;
+  long long issued = 0;
+  int maxiter = 0;
+  bool finished = false;
+  Scal* host = nullptr;  // pinned copy of the B scalar records
+  void* hin = nullptr;   // pinned instance tables (main, set-up)
+  void* hin0 = nullptr;
+  std::vector<std::pair<int, hipGraphExec_t>> graphs;
+  std::vector<hipGraph_t> graph_defs;
 };
 
+namespace {
+
 template <typename T>
-int solve_impl(cm_plan* P, int B, const cm_ccg_params* prm, const double* const* hs,
-               const uint8_t* const* surfs, void* const* us, void* const* fs, uint8_t* const* cms,
-               cm_ccg_stats* stats, cm_progress_fn progress, void* user, void* stream) {
+void enqueue_iteration(cm_session* S) {
+  const FftExec& ex = S->P->ex;
+  InstArr<T> ia{};
+  ia.arr = (const Inst<T>*)((char*)S->ws + S->L.inst);
+  const int B = S->B, nbe = S->P->nblk_elem;
+  hipStream_t st = S->st;
+  launch_elem<T, ElemPhaseA<T>>(ex, nbe, ia, B, 2, st);
+  if (S->any_bisect) hipLaunchKernelGGL((k_bisect<T>), dim3(B), dim3(1024), 0, st, ex.g, ia);
+  launch_r2c<T, ProdDir<T>>(ex, ia, B, 3, st);
+  launch_col<T, 0>(ex, ia, B, 0, 2, st);
+  launch_c2r<T, ConsTau<T>>(ex, ia, B, 4, st);
+  launch_elem<T, ElemPhaseB<T>>(ex, nbe, ia, B, 5, st);
+  launch_r2c<T, ProdForce<T>>(ex, ia, B, 6, st);
+  launch_col<T, 0>(ex, ia, B, 0, 1, st);
+  launch_c2r<T, ConsDisp<T>>(ex, ia, B, 7, st);
+}
+
+template <typename T>
+int session_begin(cm_plan* P, int B, const cm_ccg_params* prm, const double* const* hs,
+                  const uint8_t* const* surfs, void* const* us, void* const* fs, uint8_t* const* cms,
+                  void* stream, cm_session** out) {
   cm_ctx* ctx = P->ctx;
-  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
-  WsLayout L;
-  int rc = ensure_ws<T>(P, B, L);
-  if (rc) return rc;
-  char* base = (char*)P->ws;
+  for (int b = 1; b < B; ++b)
+    if (prm[b].init_mode != prm[0].init_mode)
+      return fail(ctx, CM_E_ARG, "all instances of a batch must share init_mode");
+  cm_session* S = new cm_session();
+  S->P = P;
+  S->B = B;
+  S->st = stream ? (hipStream_t)stream : ctx->stream;
+  S->L = ws_layout(P, B);
+  int rc = dev_alloc(ctx, &S->ws, S->L.total);
+  if (rc) {
+    delete S;
+    return rc;
+  }
+  S->ws_bytes = S->L.total;
+  if (hipHostMalloc((void**)&S->host, sizeof(Scal) * B, 0) != hipSuccess ||
+      hipHostMalloc(&S->hin, sizeof(Inst<T>) * B, 0) != hipSuccess ||
+      hipHostMalloc(&S->hin0, sizeof(Inst<T>) * B, 0) != hipSuccess) {
+    dev_free(ctx, S->ws, S->ws_bytes);
+    delete S;
+    return fail(ctx, CM_E_OOM, "pinned host allocation failed");
+  }
+  const WsLayout& L = S->L;
+  char* base = (char*)S->ws;
   const size_t M = (size_t)P->Mx * P->My;
-  Pinned<Inst<T>> hin(B), hin0(B);
-  Pinned<Scal> hsc(B), hout(B);
-  if (!hin.p || !hin0.p || !hsc.p || !hout.p) return fail(ctx, CM_E_OOM, "pinned host allocation failed");
-  bool any_bisect = false;
-  int maxiter = 0;
-  const int ips = prm[0].iters_per_sync;
-  int graph = prm[0].use_graph;
+  Inst<T>* hin = (Inst<T>*)S->hin;
+  Inst<T>* hin0 = (Inst<T>*)S->hin0;
+  Scal* hsc = S->host;
+  S->use_graph = prm[0].use_graph;
   for (int b = 0; b < B; ++b) {
     const cm_ccg_params& q = prm[b];
-    Inst<T>& I = hin.p[b];
+    Inst<T>& I = hin[b];
     std::memset(&I, 0, sizeof(I));
     I.u = (T*)us[b];
     I.f = (T*)fs[b];
@@ -1364,14 +1569,14 @@ int solve_impl(cm_plan* P, int B, const cm_ccg_params* prm, const double* const*
     I.ctr = (unsigned*)(base + L.ctr + b * L.st_ctr);
     // set-up table: the initial transform reads u (forces from displacements)
     // or f (displacements from initial forces) and writes the other field
-    hin0.p[b] = I;
+    hin0[b] = I;
     if (q.init_mode == 0) {
-      hin0.p[b].src = I.u;
+      hin0[b].src = I.u;
     } else {
-      hin0.p[b].src = I.f;
-      hin0.p[b].dst = I.u;
+      hin0[b].src = I.f;
+      hin0[b].dst = I.u;
     }
-    Scal& s = hsc.p[b];
+    Scal& s = hsc[b];
     std::memset(&s, 0, sizeof(Scal));
     s.F = q.external_force;
     s.H = q.hardness;
@@ -1398,19 +1603,18 @@ int solve_impl(cm_plan* P, int B, const cm_ccg_params* prm, const double* const*
       s.done = 1;
       s.status = CM_MAXITER;
     }
-    any_bisect = any_bisect || (q.has_hardness && q.load_control);
-    maxiter = std::max(maxiter, q.maxiter);
+    S->any_bisect = S->any_bisect || (q.has_hardness && q.load_control);
+    S->maxiter = std::max(S->maxiter, q.maxiter);
   }
+  hipStream_t st = S->st;
   Inst<T>* dinst = (Inst<T>*)(base + L.inst);
   Inst<T>* dinst0 = (Inst<T>*)(base + L.inst0);
   for (int b = 0; b < B; ++b)
-    HIPCHK(ctx, hipMemcpyAsync(hin.p[b].sc, &hsc.p[b], sizeof(Scal), hipMemcpyHostToDevice, st));
-  HIPCHK(ctx, hipMemcpyAsync(dinst, hin.p, sizeof(Inst<T>) * B, hipMemcpyHostToDevice, st));
-  HIPCHK(ctx, hipMemcpyAsync(dinst0, hin0.p, sizeof(Inst<T>) * B, hipMemcpyHostToDevice, st));
-  for (int b = 0; b < B; ++b) {
-    HIPCHK(ctx, hipMemsetAsync(hin.p[b].ctr, 0, 8 * sizeof(unsigned), st));
-    HIPCHK(ctx, hipMemsetAsync(hin.p[b].t, 0, M * sizeof(T), st));
-  }
+    HIPCHK(ctx, hipMemcpyAsync(hin[b].sc, &hsc[b], sizeof(Scal), hipMemcpyHostToDevice, st));
+  HIPCHK(ctx, hipMemcpyAsync(dinst, hin, sizeof(Inst<T>) * B, hipMemcpyHostToDevice, st));
+  HIPCHK(ctx, hipMemcpyAsync(dinst0, hin0, sizeof(Inst<T>) * B, hipMemcpyHostToDevice, st));
+  HIPCHK(ctx, hipMemsetAsync(base + L.ctr, 0, L.st_ctr * B, st));
+  HIPCHK(ctx, hipMemsetAsync(base + L.t, 0, L.st_t * B, st));
   InstArr<T> ia{};
   ia.arr = dinst;
   InstArr<T> ia0{};
@@ -1430,95 +1634,177 @@ int solve_impl(cm_plan* P, int B, const cm_ccg_params* prm, const double* const*
     launch_elem<T, ElemZeroPad<T>>(ex, nbe, ia, B, 1, st);
   }
   HIPCHK(ctx, hipGetLastError());
+  *out = S;
+  return 0;
+}
 
-  auto enqueue_iteration = [&]() {
+// Enqueue n iterations; instances that have finished turn the kernels into
+// no-ops.  With use_graph, chunks of iterations replay a captured hipGraph.
+template <typename T>
+int session_step(cm_session* S, int n) {
+  cm_ctx* ctx = S->P->ctx;
+  if (n <= 0) return 0;
+  if (S->use_graph && n >= 8) {
+    hipGraphExec_t gx = nullptr;
+    for (auto& g : S->graphs)
+      if (g.first == n) gx = g.second;
+    if (!gx) {
+      hipGraph_t gdef = nullptr;
+      hipError_t e = hipStreamBeginCapture(S->st, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        for (int k = 0; k < n; ++k) enqueue_iteration<T>(S);
+        e = hipStreamEndCapture(S->st, &gdef);
+      }
+      if (e == hipSuccess) e = hipGraphInstantiate(&gx, gdef, nullptr, nullptr, 0);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        if (gdef) (void)hipGraphDestroy(gdef);
+        S->use_graph = 0;  // capture unavailable on this stream: launch directly
+        gx = nullptr;
+      } else {
+        S->graphs.emplace_back(n, gx);
+        S->graph_defs.push_back(gdef);
+      }
+    }
+    if (gx) {
+      HIPCHK(ctx, hipGraphLaunch(gx, S->st));
+      S->issued += n;
+      return 0;
+    }
+  }
+  for (int k = 0; k < n; ++k) enqueue_iteration<T>(S);
+  S->issued += n;
+  HIPCHK(ctx, hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int session_finish(cm_session* S) {
+  const FftExec& ex = S->P->ex;
+  InstArr<T> ia{};
+  ia.arr = (const Inst<T>*)((char*)S->ws + S->L.inst);
+  launch_elem<T, ElemEnergy<T>>(ex, S->P->nblk_elem, ia, S->B, 0, S->st);
+  return 0;
+}
+
+// Per-kernel durations of one iteration (HIP events between launches).
+template <typename T>
+int session_profile(cm_session* S, int n, float* us) {
+  cm_ctx* ctx = S->P->ctx;
+  const FftExec& ex = S->P->ex;
+  InstArr<T> ia{};
+  ia.arr = (const Inst<T>*)((char*)S->ws + S->L.inst);
+  const int B = S->B, nbe = S->P->nblk_elem;
+  hipStream_t st = S->st;
+  constexpr int NK = 9;
+  hipEvent_t ev[NK + 1];
+  for (auto& e : ev) HIPCHK(ctx, hipEventCreate(&e));
+  std::vector<double> acc(NK, 0.0);
+  for (int it = 0; it < n; ++it) {
+    int k = 0;
+    HIPCHK(ctx, hipEventRecord(ev[k++], st));
     launch_elem<T, ElemPhaseA<T>>(ex, nbe, ia, B, 2, st);
-    if (any_bisect) hipLaunchKernelGGL((k_bisect<T>), dim3(B), dim3(1024), 0, st, ex.g, ia);
+    HIPCHK(ctx, hipEventRecord(ev[k++], st));
+    if (S->any_bisect) hipLaunchKernelGGL((k_bisect<T>), dim3(B), dim3(1024), 0, st, ex.g, ia);
+    HIPCHK(ctx, hipEventRecord(ev[k++], st));
     launch_r2c<T, ProdDir<T>>(ex, ia, B, 3, st);
+    HIPCHK(ctx, hipEventRecord(ev[k++], st));
     launch_col<T, 0>(ex, ia, B, 0, 2, st);
+    HIPCHK(ctx, hipEventRecord(ev[k++], st));
     launch_c2r<T, ConsTau<T>>(ex, ia, B, 4, st);
+    HIPCHK(ctx, hipEventRecord(ev[k++], st));
     launch_elem<T, ElemPhaseB<T>>(ex, nbe, ia, B, 5, st);
+    HIPCHK(ctx, hipEventRecord(ev[k++], st));
     launch_r2c<T, ProdForce<T>>(ex, ia, B, 6, st);
+    HIPCHK(ctx, hipEventRecord(ev[k++], st));
     launch_col<T, 0>(ex, ia, B, 0, 1, st);
+    HIPCHK(ctx, hipEventRecord(ev[k++], st));
     launch_c2r<T, ConsDisp<T>>(ex, ia, B, 7, st);
-  };
+    HIPCHK(ctx, hipEventRecord(ev[k++], st));
+    HIPCHK(ctx, hipEventSynchronize(ev[NK]));
+    for (int i = 0; i < NK; ++i) {
+      float ms = 0;
+      HIPCHK(ctx, hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+      acc[i] += ms * 1000.0;
+    }
+  }
+  S->issued += n;
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (int i = 0; i < NK; ++i) us[i] = (float)(acc[i] / std::max(1, n));
+  return 0;
+}
 
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graphh = nullptr;
+int session_poll(cm_session* S, cm_ccg_stats* stats, int* all_done) {
+  cm_ctx* ctx = S->P->ctx;
+  const char* base = (const char*)S->ws;
+  for (int b = 0; b < S->B; ++b)
+    HIPCHK(ctx, hipMemcpyAsync(&S->host[b], base + S->L.sc + b * S->L.st_sc, sizeof(Scal),
+                               hipMemcpyDeviceToHost, S->st));
+  HIPCHK(ctx, hipStreamSynchronize(S->st));
+  bool done = true;
+  for (int b = 0; b < S->B; ++b) {
+    if (stats) fill_stats(S->host[b], &stats[b]);
+    done = done && S->host[b].done;
+  }
+  if (all_done) *all_done = done ? 1 : 0;
+  return 0;
+}
+
+void session_free(cm_session* S) {
+  if (!S) return;
+  cm_ctx* ctx = S->P->ctx;
+  (void)hipStreamSynchronize(S->st);
+  for (auto& g : S->graphs) (void)hipGraphExecDestroy(g.second);
+  for (auto& g : S->graph_defs) (void)hipGraphDestroy(g);
+  dev_free(ctx, S->ws, S->ws_bytes);
+  if (S->host) (void)hipHostFree(S->host);
+  if (S->hin) (void)hipHostFree(S->hin);
+  if (S->hin0) (void)hipHostFree(S->hin0);
+  delete S;
+}
+
+// Whole solve: set-up, chunks of iterations with a host check of the done
+// flags after each chunk (chunks double up to 64), energy, statistics.
+int solve_driver(cm_plan* P, int B, const cm_ccg_params* prm, const double* const* hs,
+                 const uint8_t* const* surfs, void* const* us, void* const* fs, uint8_t* const* cms,
+                 cm_ccg_stats* stats, cm_progress_fn progress, void* user, void* stream) {
+  cm_session* S = nullptr;
+  int rc = P->prec == CM_FP64 ? session_begin<double>(P, B, prm, hs, surfs, us, fs, cms, stream, &S)
+                              : session_begin<float>(P, B, prm, hs, surfs, us, fs, cms, stream, &S);
+  if (rc) return rc;
+  const int ips = prm[0].iters_per_sync;
   int chunk = ips > 0 ? ips : 4;
-  int graph_chunk = 0;
-  long long issued = 0;
+  std::vector<cm_ccg_stats> tmp(B);
   int result = 0;
   while (true) {
-    bool launched = false;
-    if (graph && ips <= 0 && chunk >= 16) {
-      if (!gexec || graph_chunk != chunk) {
-        if (gexec) {
-          (void)hipGraphExecDestroy(gexec);
-          (void)hipGraphDestroy(graphh);
-          gexec = nullptr;
-          graphh = nullptr;
-        }
-        hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-        if (e == hipSuccess) {
-          for (int k = 0; k < chunk; ++k) enqueue_iteration();
-          e = hipStreamEndCapture(st, &graphh);
-        }
-        if (e == hipSuccess) e = hipGraphInstantiate(&gexec, graphh, nullptr, nullptr, 0);
-        if (e == hipSuccess) {
-          graph_chunk = chunk;
-        } else {
-          graph = 0;
-          gexec = nullptr;
-          (void)hipGetLastError();
-        }
-      }
-      if (gexec) {
-        HIPCHK(ctx, hipGraphLaunch(gexec, st));
-        launched = true;
-      }
-    }
-    if (!launched)
-      for (int k = 0; k < chunk; ++k) enqueue_iteration();
-    issued += chunk;
-    for (int b = 0; b < B; ++b)
-      (void)hipMemcpyAsync(&hout.p[b], hin.p[b].sc, sizeof(Scal), hipMemcpyDeviceToHost, st);
-    hipError_t e = hipStreamSynchronize(st);
-    if (e != hipSuccess) {
-      result = fail(ctx, CM_E_HIP, std::string("solve: ") + hipGetErrorString(e));
+    rc = P->prec == CM_FP64 ? session_step<double>(S, chunk) : session_step<float>(S, chunk);
+    if (rc) {
+      result = rc;
       break;
     }
-    if (progress) {
-      cm_ccg_stats tmp{};
-      fill_stats(hout.p[0], &tmp);
-      progress(user, &tmp);
+    int done = 0;
+    if ((rc = session_poll(S, tmp.data(), &done))) {
+      result = rc;
+      break;
     }
-    bool all_done = true;
-    for (int b = 0; b < B; ++b) all_done = all_done && hout.p[b].done;
-    if (all_done) break;
-    if (issued > (long long)maxiter + 8) {
-      result = fail(ctx, CM_E_ARG, "solver did not terminate within maxiter");
+    if (progress) progress(user, &tmp[0]);
+    if (done) break;
+    if (S->issued > (long long)S->maxiter + 8) {
+      result = fail(P->ctx, CM_E_ARG, "solver did not terminate within maxiter");
       break;
     }
     if (ips <= 0) chunk = std::min(chunk * 2, 64);
   }
-  if (gexec) {
-    (void)hipGraphExecDestroy(gexec);
-    (void)hipGraphDestroy(graphh);
+  if (result == 0) {
+    P->prec == CM_FP64 ? session_finish<double>(S) : session_finish<float>(S);
+    if ((rc = session_poll(S, stats, nullptr))) result = rc;
   }
   if (result == 0) {
-    launch_elem<T, ElemEnergy<T>>(ex, nbe, ia, B, 0, st);
-    for (int b = 0; b < B; ++b)
-      (void)hipMemcpyAsync(&hout.p[b], hin.p[b].sc, sizeof(Scal), hipMemcpyDeviceToHost, st);
-    hipError_t e = hipStreamSynchronize(st);
-    if (e != hipSuccess) result = fail(ctx, CM_E_HIP, std::string("finalize: ") + hipGetErrorString(e));
-  }
-  if (result == 0) {
-    for (int b = 0; b < B; ++b) fill_stats(hout.p[b], &stats[b]);
     result = stats[0].status;
     for (int b = 1; b < B; ++b)
       if (stats[b].status < 0) result = stats[b].status;
   }
+  session_free(S);
   return result;
 }
 
@@ -1635,9 +1921,7 @@ int cm_ccg_solve(cm_plan* plan, const cm_ccg_params* params, const double* heigh
   void* us[1] = {u};
   void* fs[1] = {f};
   uint8_t* cs[1] = {cmask};
-  return plan->prec == CM_FP64
-             ? solve_impl<double>(plan, 1, params, hs, ms, us, fs, cs, stats, progress, user, stream)
-             : solve_impl<float>(plan, 1, params, hs, ms, us, fs, cs, stats, progress, user, stream);
+  return solve_driver(plan, 1, params, hs, ms, us, fs, cs, stats, progress, user, stream);
 }
 
 int cm_ccg_solve_batch(cm_plan* plan, int B, const cm_ccg_params* params, const double* const* heights,
@@ -1645,12 +1929,49 @@ int cm_ccg_solve_batch(cm_plan* plan, int B, const cm_ccg_params* params, const 
                        cm_ccg_stats* stats, void* stream) {
   if (!plan || B < 1 || !params || !heights || !u || !f || !cmask || !stats)
     return fail(plan ? plan->ctx : nullptr, CM_E_ARG, "bad argument");
-  for (int b = 1; b < B; ++b)
-    if (params[b].init_mode != params[0].init_mode)
-      return fail(plan->ctx, CM_E_ARG, "all instances of a batch must share init_mode");
+  return solve_driver(plan, B, params, heights, surf_masks, u, f, cmask, stats, nullptr, nullptr, stream);
+}
+
+size_t cm_ccg_session_bytes(cm_plan* plan, int B) {
+  if (!plan || B < 1) return 0;
+  return ws_layout(plan, B).total;
+}
+
+int cm_ccg_begin(cm_plan* plan, int B, const cm_ccg_params* params, const double* const* heights,
+                 const uint8_t* const* surf_masks, void* const* u, void* const* f, uint8_t* const* cmask,
+                 void* stream, cm_session** out) {
+  if (!plan || B < 1 || !params || !heights || !u || !f || !cmask || !out)
+    return fail(plan ? plan->ctx : nullptr, CM_E_ARG, "bad argument");
   return plan->prec == CM_FP64
-             ? solve_impl<double>(plan, B, params, heights, surf_masks, u, f, cmask, stats, nullptr, nullptr, stream)
-             : solve_impl<float>(plan, B, params, heights, surf_masks, u, f, cmask, stats, nullptr, nullptr, stream);
+             ? session_begin<double>(plan, B, params, heights, surf_masks, u, f, cmask, stream, out)
+             : session_begin<float>(plan, B, params, heights, surf_masks, u, f, cmask, stream, out);
+}
+
+int cm_ccg_step(cm_session* s, int n_iters) {
+  if (!s) return fail(nullptr, CM_E_ARG, "null session");
+  return s->P->prec == CM_FP64 ? session_step<double>(s, n_iters) : session_step<float>(s, n_iters);
+}
+
+int cm_ccg_poll(cm_session* s, cm_ccg_stats* stats, int* all_done) {
+  if (!s) return fail(nullptr, CM_E_ARG, "null session");
+  return session_poll(s, stats, all_done);
+}
+
+int cm_ccg_finish(cm_session* s, cm_ccg_stats* stats) {
+  if (!s || !stats) return fail(nullptr, CM_E_ARG, "null argument");
+  s->P->prec == CM_FP64 ? session_finish<double>(s) : session_finish<float>(s);
+  return session_poll(s, stats, nullptr);
+}
+
+int cm_ccg_end(cm_session* s) {
+  session_free(s);
+  return 0;
+}
+
+int cm_ccg_profile(cm_session* s, int n_iters, float* kernel_us) {
+  if (!s || !kernel_us || n_iters < 1) return fail(nullptr, CM_E_ARG, "bad argument");
+  return s->P->prec == CM_FP64 ? session_profile<double>(s, n_iters, kernel_us)
+                               : session_profile<float>(s, n_iters, kernel_us);
 }
 
 }  // extern "C"

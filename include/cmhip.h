@@ -48,6 +48,7 @@ extern "C" {
 
 typedef struct cm_ctx cm_ctx;
 typedef struct cm_plan cm_plan;
+typedef struct cm_session cm_session;
 
 /* Substrate description.  Replaces the constructor arguments of
  * PeriodicFFTElasticHalfSpace (FFTElasticHalfSpace.py:169-306) and
@@ -147,6 +148,29 @@ int cm_ccg_solve_batch(cm_plan* plan, int B, const cm_ccg_params* params,
                        const double* const* heights, const uint8_t* const* surf_masks,
                        void* const* u, void* const* f, uint8_t* const* cmask,
                        cm_ccg_stats* stats, void* stream);
+
+/* ---- resident solves (serving, batching, benchmarking) ----------------
+ * A session keeps B instances' solver state (conjugate directions, spectra,
+ * scalar state) resident on the device.  cm_ccg_begin runs the set-up
+ * (ConstrainedConjugateGradients.py:179-219) asynchronously on `stream`;
+ * cm_ccg_step enqueues n iterations (finished instances turn into no-ops);
+ * cm_ccg_poll synchronises and reads the statistics; cm_ccg_finish computes
+ * the elastic energy; cm_ccg_end frees the workspace.  Sessions on different
+ * streams run concurrently.  No counterpart in the reference, which only has
+ * the blocking call replaced by cm_ccg_solve. */
+size_t cm_ccg_session_bytes(cm_plan* plan, int B);
+int cm_ccg_begin(cm_plan* plan, int B, const cm_ccg_params* params, const double* const* heights,
+                 const uint8_t* const* surf_masks, void* const* u, void* const* f, uint8_t* const* cmask,
+                 void* stream, cm_session** out);
+int cm_ccg_step(cm_session* s, int n_iters);
+int cm_ccg_poll(cm_session* s, cm_ccg_stats* stats, int* all_done);
+int cm_ccg_finish(cm_session* s, cm_ccg_stats* stats);
+int cm_ccg_end(cm_session* s);
+/* Run n iterations launch by launch with HIP events between the 9 kernels of
+ * an iteration; kernel_us[9] receives the mean duration of each (phase A,
+ * bisection, direction r2c, column, tau c2r, phase B, force r2c, column,
+ * displacement c2r).  Measurement hook, no reference counterpart. */
+int cm_ccg_profile(cm_session* s, int n_iters, float* kernel_us);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,102 @@
+"""Batched uneven solves, the request server and the load sweep on the GPU.
+
+* a batch of instances of different sizes gives, per instance, bit-identical
+  forces to the same instance solved alone (own reduction tree per instance);
+* the server gives bit-identical results for a request alone and in a mix of
+  concurrent requests, reports latencies, and refuses requests above budget;
+* the load sweep equals independent oracle solves per load.
+"""
+
+import numpy as np
+import pytest
+
+from contactmechanics_amd import workloads
+from contactmechanics_amd.batch import PlanCache, solve_batch
+from contactmechanics_amd.serve import ContactServer, request_bytes
+from oracle.ccg import constrained_conjugate_gradients as ccg_cpu
+from oracle.halfspace import FreeHalfSpace, PeriodicHalfSpace
+
+pytestmark = pytest.mark.gpu
+
+
+def problem(n, bc, seed, p=3e-3, hardness=None):
+    h = workloads.self_affine(n, 0.8, 0.1, seed)
+    tol = workloads.tolerances(1e-9, n, n)
+    return dict(n=n, bc=bc, heights=h, external_force=p, hardness=hardness, maxiter=5000, **tol)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_equals_alone(gpu, precision):
+    probs = [problem(64, "periodic", 1), problem(96, "periodic", 2), problem(64, "periodic", 3),
+             problem(48, "free", 4), problem(64, "free", 5, hardness=0.02 / (64 * 64))]
+    cache = PlanCache(precision)
+    res, info = solve_batch(probs, precision=precision, cache=cache)
+    assert info["groups"] == 4
+    for i, p in enumerate(probs):
+        alone, _ = solve_batch([p], precision=precision, cache=cache)
+        assert res[i]["stats"]["status"] == 0
+        assert res[i]["stats"]["nit"] == alone[0]["stats"]["nit"]
+        assert bool((res[i]["jac"] == alone[0]["jac"]).all())
+
+
+def test_batch_matches_oracle(gpu):
+    probs = [problem(64, "periodic", 11), problem(32, "free", 12)]
+    res, _ = solve_batch(probs, precision="fp64")
+    for p, r in zip(probs, res):
+        cls = PeriodicHalfSpace if p["bc"] == "periodic" else FreeHalfSpace
+        ref = ccg_cpu(cls((p["n"], p["n"]), 1.0, (1.0, 1.0)), p["heights"], external_force=p["external_force"],
+                      pentol=p["pentol"], forcetol=p["forcetol"], maxiter=5000)
+        jac = r["jac"].cpu().numpy()
+        assert np.max(np.abs(jac - ref.jac)) / np.max(np.abs(ref.jac)) < 1e-6
+
+
+def test_server_mix_equals_alone_and_budget(gpu):
+    reqs = []
+    for i, (n, bc) in enumerate([(64, "periodic"), (128, "periodic"), (64, "free"), (96, "periodic"),
+                                 (32, "free"), (128, "free")]):
+        p = problem(n, bc, 100 + i)
+        p.update(index=i, arrival=0.0005 * i)
+        reqs.append(p)
+    got = {}
+
+    def keep(rec, s):
+        got[(rec["index"], "mix")] = s.pressures(0).cpu().numpy().copy()
+
+    srv = ContactServer(precision="fp32", streams=4)
+    done, summary = srv.run(reqs, on_done=keep)
+    assert summary["requests"] == len(reqs) and summary["rejected"] == 0
+    assert summary["p99_latency_s"] >= summary["p50_latency_s"] > 0
+    for r in reqs:
+        alone = {}
+
+        def keep1(rec, s):
+            alone["f"] = s.pressures(0).cpu().numpy().copy()
+
+        ContactServer(precision="fp32", streams=1).run([dict(r, arrival=0.0)], on_done=keep1)
+        assert np.array_equal(alone["f"], got[(r["index"], "mix")])
+    # a request larger than the whole budget is refused, the rest still run
+    sub = PlanCache("fp32").get(128, "free")
+    small = ContactServer(precision="fp32", mem_budget=request_bytes(sub) - 1)
+    done, summary = small.run([dict(reqs[5], arrival=0.0), dict(reqs[0], arrival=0.0)])
+    assert summary["rejected"] == 1 and summary["requests"] == 1
+
+
+def test_load_sweep_pressures_and_plastic(gpu):
+    from contactmechanics_amd.PipelineFunction import contact_mechanics
+    from contactmechanics_amd import Topography
+    n = 64
+    h = workloads.self_affine(n, 0.7, 0.2, 33)
+    topo = Topography(h, (1.0, 1.0))
+    loads = [1e-3, 3e-3]
+    mp, area, disp, gap, conv = contact_mechanics(topo, substrate="periodic", pressures=loads, maxiter=2000)
+    assert conv.all()
+    np.testing.assert_allclose(mp, loads, rtol=1e-6)
+    hs = PeriodicHalfSpace((n, n), 1.0, (1.0, 1.0))
+    pentol = max(np.std(h) / (10 * n), 1e-12)
+    for p, a in zip(loads, area):
+        ref = ccg_cpu(hs, h, external_force=p, pentol=pentol, maxiter=2000)
+        assert abs(a - np.mean(ref.jac > 0)) < 2e-3
+    mp2, area2, _, _, conv2 = contact_mechanics(topo, substrate="periodic", pressures=loads, hardness=0.05,
+                                                maxiter=2000, warm_start=True)
+    assert conv2.all()
+    assert np.all(area2 >= area - 1e-12)  # a hardness cap can only spread the contact
