@@ -122,6 +122,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=30, help="oracle iterations for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a plain 256 MiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -133,6 +135,13 @@ def main():
     from contactmechanics_amd import PeriodicFFTElasticHalfSpace, workloads
 
     torch.cuda.set_device(0)
+    if args.calib_copy:
+        src = torch.ones(32 * 2 ** 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print("# calibration copy: 268435456 bytes read + written per launch", file=sys.stderr, flush=True)
     cfg = workloads.config2(args.n)
     n = cfg["n"]
     M = Mt = n * n

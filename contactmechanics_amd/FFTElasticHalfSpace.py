@@ -15,7 +15,10 @@ reference) or torch tensors on the HIP device (returned as device tensors in
 the plan precision, no host copies).
 
 Build-only keyword: ``precision="fp64"`` (default, every field and transform in
-FP64) or ``"fp32"`` (FP32 fields and transforms, FP64 reductions and scalars).
+FP64), ``"fp32"`` (FP32 fields and transforms, FP64 reductions and scalars) or
+``"mixed"`` (operators in FP64; the solver runs its bulk iterations on an FP32
+twin of the plan and finishes with FP64 iterations warm-started from the FP32
+forces, see ``Optimization/ConstrainedConjugateGradients.py``).
 """
 
 import numpy as np
@@ -66,11 +69,12 @@ class PeriodicFFTElasticHalfSpace(ElasticSubstrate):
         self.stiffness_q0 = stiffness_q0
         self.thickness = thickness
         self._communicator = communicator
-        if precision not in ("fp64", "fp32"):
-            raise self.Error("precision must be 'fp64' or 'fp32'")
+        if precision not in ("fp64", "fp32", "mixed"):
+            raise self.Error("precision must be 'fp64', 'fp32' or 'mixed'")
         self.precision = precision
         self._device = device
         self._plan = None
+        self._twin = None
 
     # ---- grid properties (reference :308-430)
     @property
@@ -159,7 +163,7 @@ class PeriodicFFTElasticHalfSpace(ElasticSubstrate):
         """The device plan (built on first use: needs a HIP device)."""
         if self._plan is None:
             ctx = _native.Context.get(self._device)
-            prec = _native.CM_FP64 if self.precision == "fp64" else _native.CM_FP32
+            prec = _native.CM_FP32 if self.precision == "fp32" else _native.CM_FP64
             nx, ny = self.nb_grid_pts
             sx, sy = self.physical_sizes
             try:
@@ -172,7 +176,18 @@ class PeriodicFFTElasticHalfSpace(ElasticSubstrate):
     @property
     def torch_dtype(self):
         import torch
-        return torch.float64 if self.precision == "fp64" else torch.float32
+        return torch.float32 if self.precision == "fp32" else torch.float64
+
+    def fp32_twin(self):
+        """The same substrate with an FP32 plan (bulk iterations of "mixed")."""
+        if self._twin is None:
+            import copy
+            twin = copy.copy(self)
+            twin.precision = "fp32"
+            twin._plan = None
+            twin._twin = None
+            self._twin = twin
+        return self._twin
 
     def _device_str(self):
         import torch

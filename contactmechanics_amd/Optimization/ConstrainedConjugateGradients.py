@@ -17,6 +17,13 @@ Extra keywords (not in the reference):
     ``logger`` forces 1 (the host then sees every iteration).
 ``use_graph``
     replay chunks of iterations from a captured hipGraph.
+
+Substrates built with ``precision="mixed"`` solve in two stages: FP32 bulk
+iterations (fields and transforms FP32, reductions and scalars FP64) to
+tolerances FP32 can resolve (``pentol``, ``forcetol`` floored at 1e-6 of the
+RMS height and of the mean pixel force), then FP64 iterations started from the
+FP32 forces (``initial_forces``) to the requested tolerances.  The result has
+``nit`` = total iterations and ``nit_fp32`` / ``nit_fp64`` per stage.
 """
 
 import ctypes
@@ -81,6 +88,13 @@ def constrained_conjugate_gradients(substrate, topography, hardness=None, extern
                         f"dimensions (Your substrate has {substrate.dim}.).")
     if not hasattr(substrate, "plan"):
         raise TypeError("substrate must be a contactmechanics_amd FFT half-space")
+    if getattr(substrate, "precision", None) == "mixed" and initial_forces is None:
+        return _mixed_precision(substrate, topography, hardness=hardness, external_force=external_force,
+                                offset=offset, initial_displacements=initial_displacements, pentol=pentol,
+                                forcetol=forcetol, thermotol=thermotol, mixfac=mixfac, mixdecfac=mixdecfac,
+                                minmixsteps=minmixsteps, maxiter=maxiter, logger=logger, callback=callback,
+                                return_device=return_device, iters_per_sync=iters_per_sync,
+                                use_graph=use_graph)
 
     h_np, valid, heights = _heights_and_mask(topography)
     nx, ny = substrate.nb_grid_pts
@@ -227,3 +241,32 @@ def _assemble_result(substrate, stats, u, f, cmask, hardness, return_device, log
 DOIS = ("10.1016/S0043-1648(99)00113-1",)
 PLASTIC_DOIS = ("10.1016/j.triboint.2005.11.008", "10.1038/s41467-018-02981-y")
 
+
+
+def _mixed_precision(substrate, topography, pentol=None, forcetol=1e-5, thermotol=1e-6, external_force=None,
+                     maxiter=100000, return_device=False, **kw):
+    """FP32 bulk iterations, then FP64 iterations from the FP32 forces."""
+    import torch
+    h_np, valid, heights = _heights_and_mask(topography)
+    nx, ny = substrate.nb_grid_pts
+    if pentol is None:
+        pentol = _rms_height(heights) / (10 * np.mean(substrate.nb_grid_pts))
+    rms = _rms_height(heights)
+    scale_f = abs(external_force) / (nx * ny) if external_force is not None else 0.0
+    twin = substrate.fp32_twin()
+    r32 = constrained_conjugate_gradients(
+        twin, topography, external_force=external_force, pentol=max(pentol, 1e-6 * rms),
+        forcetol=max(forcetol, 1e-6 * scale_f), thermotol=max(thermotol, 1e-6), maxiter=maxiter,
+        return_device=True, **kw)
+    f0 = torch.zeros(tuple(substrate.nb_subdomain_grid_pts), dtype=torch.float64, device=substrate._device_str())
+    f0[:nx, :ny] = -r32.jac.to(torch.float64)
+    kw.pop("initial_displacements", None)
+    r64 = constrained_conjugate_gradients(
+        substrate, topography, external_force=external_force, pentol=pentol, forcetol=forcetol,
+        thermotol=thermotol, maxiter=max(1, maxiter - int(r32.nit)), initial_forces=f0,
+        return_device=return_device, **kw)
+    r64.nit_fp32 = int(r32.nit)
+    r64.nit_fp64 = int(r64.nit)
+    r64.nit = r64.nit_fp32 + r64.nit_fp64
+    r64.nfev = int(r64.nfev) + int(r32.nfev)
+    return r64

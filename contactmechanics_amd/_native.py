@@ -92,6 +92,9 @@ SIGNATURES = {
     "cm_plan_destroy": ([_VP], ctypes.c_int),
     "cm_plan_grid": ([_VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "cm_plan_green_host": ([_VP, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "cm_plan_autotune": ([_VP, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t], ctypes.c_int),
+    "cm_plan_set_shape": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "cm_plan_shape": ([_VP] + [ctypes.POINTER(ctypes.c_int)] * 4, ctypes.c_int),
     "cm_evaluate_disp": ([_VP, _VP, _VP, _VP], ctypes.c_int),
     "cm_evaluate_force": ([_VP, _VP, _VP, _VP], ctypes.c_int),
     "cm_ccg_solve": ([_VP, ctypes.POINTER(CcgParams), _VP, _VP, _VP, _VP, _VP,
@@ -233,6 +236,20 @@ class Plan:
         check(self.lib.cm_plan_green_host(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
               self.ctx.handle)
         return out
+
+    def autotune(self, reps=10):
+        buf = ctypes.create_string_buffer(512)
+        check(self.lib.cm_plan_autotune(self.handle, int(reps), buf, 512), self.ctx.handle)
+        import json
+        return json.loads(buf.value.decode())
+
+    def set_shape(self, Cb, R, erow, ecol):
+        check(self.lib.cm_plan_set_shape(self.handle, int(Cb), int(R), int(erow), int(ecol)), self.ctx.handle)
+
+    def shape(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        check(self.lib.cm_plan_shape(self.handle, *[ctypes.byref(x) for x in v]), self.ctx.handle)
+        return dict(zip(["Cb", "R", "erow", "ecol"], [x.value for x in v]))
 
     def evaluate_disp(self, f_ptr, u_ptr):
         st = self.ctx.run_begin()

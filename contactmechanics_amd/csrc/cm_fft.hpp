@@ -236,6 +236,125 @@ __device__ void fft_line(cplx<T>* lds, int base, int stride, const LinePlan& p, 
 }
 
 // ---------------------------------------------------------------------------
+// Register-ended passes.  "First-pass order" for radix R: a thread's register
+// slot e = b*R + r holds element tl + b*tpl + r*(L/R) (b < nb butterflies);
+// "last-pass order" is the same map with the last pass's radix.  Feeding the
+// first pass from registers and leaving the last pass in registers lets
+// kernels read inputs from / write outputs to HBM without an LDS round trip.
+
+__device__ __forceinline__ int nbutterflies(int L, int R, int tl, int tpl) {
+  const int nbf = L / R;
+  return tl < nbf ? (nbf - tl + tpl - 1) / tpl : 0;
+}
+
+// Element index held by register slot e for a pass of radix R.
+__device__ __forceinline__ int slot_element(int e, int R, int L, int tl, int tpl) {
+  return tl + (e / R) * tpl + (e % R) * (L / R);
+}
+
+template <typename T, int R, bool INV, int EMAX>
+__device__ __forceinline__ void first_pass_regs(const cplx<T>* v, cplx<T>* lds, int base, int stride, int L,
+                                                int tl, int tpl) {
+  constexpr int NBMAX = EMAX / R > 0 ? EMAX / R : 1;
+  const int nb = nbutterflies(L, R, tl, tpl);
+#pragma unroll
+  for (int b = 0; b < NBMAX; ++b) {
+    if (b < nb) {
+      cplx<T> w[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) w[r] = v[b * R + r];
+      Dft<T, R, INV>::run(w);
+      const int o = (tl + b * tpl) * R;  // Ns = 1
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[lpad<T>(base + (o + r) * stride)] = w[r];
+    }
+  }
+}
+
+template <typename T, int R, bool INV, int EMAX>
+__device__ __forceinline__ void last_pass_regs(cplx<T>* v, const cplx<T>* lds, int base, int stride, int L,
+                                               int tl, int tpl, const cplx<T>* __restrict__ tw) {
+  constexpr int NBMAX = EMAX / R > 0 ? EMAX / R : 1;
+  const int nb = nbutterflies(L, R, tl, tpl);
+  const int LR = L / R;  // = Ns of the last pass
+#pragma unroll
+  for (int b = 0; b < NBMAX; ++b) {
+    if (b < nb) {
+      const int j = tl + b * tpl;
+      cplx<T> w[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) w[r] = lds[lpad<T>(base + (j + r * LR) * stride)];
+      if (LR > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          cplx<T> t = tw[r * j];
+          w[r] = INV ? cmulc(w[r], t) : cmul(w[r], t);
+        }
+      }
+      Dft<T, R, INV>::run(w);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b * R + r] = w[r];
+    }
+  }
+}
+
+// Line transform with the input in registers (IN_REGS, first-pass order) or
+// in LDS (natural order), and the output left in registers (OUT_REGS,
+// last-pass order) or in LDS (natural order).  With OUT_REGS the caller must
+// synchronise before writing LDS again.
+template <typename T, bool INV, int EMAX, bool IN_REGS, bool OUT_REGS>
+__device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const LinePlan& p, int tl,
+                           const cplx<T>* __restrict__ tw) {
+  int s0 = 0, s1 = p.npass, Ns = 1;
+  if (IN_REGS) {
+    switch (p.radix[0]) {
+      case 2: first_pass_regs<T, 2, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
+      case 3: first_pass_regs<T, 3, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
+      case 4: first_pass_regs<T, 4, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
+      case 5: first_pass_regs<T, 5, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
+      case 16: first_pass_regs<T, 16, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
+      default: first_pass_regs<T, 8, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
+    }
+    __syncthreads();
+    Ns = p.radix[0];
+    s0 = 1;
+  }
+  if (OUT_REGS) s1 = p.npass - 1;
+  for (int s = s0; s < s1; ++s) {
+    const int R = p.radix[s];
+    const int nb = nbutterflies(p.L, R, tl, p.tpl);
+    switch (R) {
+      case 2: stockham_pass<T, 2, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
+      case 3: stockham_pass<T, 3, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
+      case 4: stockham_pass<T, 4, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
+      case 5: stockham_pass<T, 5, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
+      case 16: stockham_pass<T, 16, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
+      default: stockham_pass<T, 8, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
+    }
+    Ns *= R;
+  }
+  if (OUT_REGS) {
+    if (IN_REGS && p.npass == 1) {
+      // the single pass already wrote natural order to LDS
+      const int R = p.radix[0];
+      const int nb = nbutterflies(p.L, R, tl, p.tpl);
+#pragma unroll
+      for (int e = 0; e < EMAX; ++e)
+        if (e / R < nb) v[e] = lds[lpad<T>(base + slot_element(e, R, p.L, tl, p.tpl) * stride)];
+      return;
+    }
+    switch (p.radix[p.npass - 1]) {
+      case 2: last_pass_regs<T, 2, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
+      case 3: last_pass_regs<T, 3, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
+      case 4: last_pass_regs<T, 4, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
+      case 5: last_pass_regs<T, 5, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
+      case 16: last_pass_regs<T, 16, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
+      default: last_pass_regs<T, 8, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Reductions in FP64.  A reduction record holds NS slots; OPS[i] is the op.
 enum RedOp : int { RSUM = 0, RMAX = 1, RMIN = 2 };
 

@@ -27,6 +27,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/cmhip.h"
@@ -70,7 +71,7 @@ struct Inst {
   T* f;            // forces (M grid)
   T* t;            // conjugate direction (M grid)
   uint8_t* cm;     // bit0 contact set c, bit1 gap<0 at start of iteration
-  const double* h; // heights (nx*ny)
+  const T* h;      // heights (nx*ny), in the plan precision
   const uint8_t* surf;  // valid-pixel mask (nx*ny) or null
   cplx<T>* S;      // spectrum work buffer
   const T* G;      // evaluate_disp kernel, (N2+1) x Mx
@@ -119,11 +120,11 @@ __device__ bool reduce_blocks(double* acc, double* part, unsigned* ctr, int nblo
   double v[NS > 0 ? NS : 1];
 #pragma unroll
   for (int i = 0; i < NS; ++i) v[i] = red_identity(ops[i]);
+  // plain loads: the acquire fence above makes every block's partials
+  // visible, and independent loads can all be in flight at once
   for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
 #pragma unroll
-    for (int i = 0; i < NS; ++i)
-      v[i] = red_apply(ops[i], v[i],
-                       __hip_atomic_load(&part[(size_t)b * NS + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    for (int i = 0; i < NS; ++i) v[i] = red_apply(ops[i], v[i], part[(size_t)b * NS + i]);
   }
   block_reduce<NS>(v, ops, scr);
   if (threadIdx.x == 0) {
@@ -177,7 +178,8 @@ struct HPair {
   double h0, h1;
   bool c0, c1;  // in the computational region
 };
-__device__ __forceinline__ HPair ldh(const Geom& g, const double* h, const uint8_t* surf, int row, int col) {
+template <typename T>
+__device__ __forceinline__ HPair ldh(const Geom& g, const T* h, const uint8_t* surf, int row, int col) {
   HPair r;
   r.c0 = row < g.nx && col < g.ny;
   r.c1 = row < g.nx && col + 1 < g.ny;
@@ -186,8 +188,8 @@ __device__ __forceinline__ HPair ldh(const Geom& g, const double* h, const uint8
     r.c0 = r.c0 && surf[q];
     r.c1 = r.c1 && surf[q + 1];
   }
-  r.h0 = r.c0 ? h[q] : 0.0;
-  r.h1 = r.c1 ? h[q + 1] : 0.0;
+  r.h0 = r.c0 ? (double)h[q] : 0.0;
+  r.h1 = r.c1 ? (double)h[q + 1] : 0.0;
   return r;
 }
 
@@ -208,25 +210,31 @@ __global__ void __launch_bounds__(256) k_r2c(Geom g, LinePlan lp, const cplx<T>*
   for (int i = 0; i < P::NS; ++i) acc[i] = red_identity(P::op(i));
   const int rb = r * N2;
   auto L = [&](int n) -> cplx<T>& { return lds[lpad<T>(rb + n)]; };
-  for (int n0 = tl; n0 < N2; n0 += kPairs * lp.tpl) {
-    typename P::Rec rec[kPairs];
+  // the producer feeds the first FFT pass directly (first-pass order)
+  cplx<T> v[EMAX];
+  {
+    const int R0 = lp.radix[0];
+    const int nb0 = nbutterflies(N2, R0, tl, lp.tpl);
 #pragma unroll
-    for (int k = 0; k < kPairs; ++k) {
-      const int n = n0 + k * lp.tpl;
-      if (n < N2) rec[k] = p.fetch(g, row, 2 * n);
-    }
+    for (int e0 = 0; e0 < EMAX; e0 += kPairs) {
+      typename P::Rec rec[kPairs];
 #pragma unroll
-    for (int k = 0; k < kPairs; ++k) {
-      const int n = n0 + k * lp.tpl;
-      if (n < N2) {
-        T a, b;
-        p.make(g, row, 2 * n, rec[k], a, b, acc);
-        L(n) = mk(a, b);
+      for (int k = 0; k < kPairs; ++k) {
+        const int e = e0 + k;
+        if (e / R0 < nb0) rec[k] = p.fetch(g, row, 2 * slot_element(e, R0, N2, tl, lp.tpl));
+      }
+#pragma unroll
+      for (int k = 0; k < kPairs; ++k) {
+        const int e = e0 + k;
+        if (e / R0 < nb0) {
+          T a, b;
+          p.make(g, row, 2 * slot_element(e, R0, N2, tl, lp.tpl), rec[k], a, b, acc);
+          v[e] = mk(a, b);
+        }
       }
     }
   }
-  __syncthreads();
-  fft_line<T, false, EMAX>(lds, r * N2, 1, lp, tl, true, tw);
+  fft_line_x<T, false, EMAX, true, false>(v, lds, rb, 1, lp, tl, tw);
   // split the half-length transform into the real-input spectrum X[k],
   // k = 0..N2, packing X[0] (real) and X[N2] (real) into slot 0.
   for (int k = tl; k <= N2 / 2; k += lp.tpl) {
@@ -276,6 +284,31 @@ __global__ void __launch_bounds__(256) k_col(Geom g, LinePlan lp, const cplx<T>*
   const int Mx = g.Mx, Cb = g.Cb;
   const size_t base = (size_t)cb * Mx * Cb;
   const int n = Mx * Cb;
+  if (MODE == 0 && cb != 0) {
+    // every column is an ordinary one: HBM -> first forward pass, last
+    // forward pass -> x G in registers -> first inverse pass (same radix, same
+    // slots), last inverse pass -> HBM.  Two LDS round trips fewer each way.
+    const int c = threadIdx.x % Cb, tl = threadIdx.x / Cb;
+    const int R0 = lp.radix[0], Rl = lp.radix[lp.npass - 1];
+    const int nb0 = nbutterflies(Mx, R0, tl, lp.tpl), nbl = nbutterflies(Mx, Rl, tl, lp.tpl);
+    cplx<T> v[EMAX];
+#pragma unroll
+    for (int e = 0; e < EMAX; ++e)
+      if (e / R0 < nb0) v[e] = S[base + (size_t)slot_element(e, R0, Mx, tl, lp.tpl) * Cb + c];
+    fft_line_x<T, false, EMAX, true, true>(v, lds, c, Cb, lp, tl, tw);
+    __syncthreads();
+    const T* Gk = G + (size_t)(cb * Cb + c) * Mx;
+#pragma unroll
+    for (int e = 0; e < EMAX; ++e)
+      if (e / Rl < nbl) v[e] = v[e] * Gk[slot_element(e, Rl, Mx, tl, lp.tpl)];
+    LinePlan rv = lp;  // inverse radices reversed: its first pass takes the slots as they are
+    for (int q = 0; q < lp.npass; ++q) rv.radix[q] = lp.radix[lp.npass - 1 - q];
+    fft_line_x<T, true, EMAX, true, true>(v, lds, c, Cb, rv, tl, tw);
+#pragma unroll
+    for (int e = 0; e < EMAX; ++e)
+      if (e / R0 < nb0) S[base + (size_t)slot_element(e, R0, Mx, tl, lp.tpl) * Cb + c] = v[e];
+    return;
+  }
   for (int i = threadIdx.x; i < n; i += blockDim.x) lds[lpad<T>(i)] = S[base + i];
   __syncthreads();
   const int c = threadIdx.x % Cb, tl = threadIdx.x / Cb;
@@ -346,23 +379,26 @@ __global__ void __launch_bounds__(256) k_c2r(Geom g, LinePlan lp, const cplx<T>*
     }
   }
   __syncthreads();
-  fft_line<T, true, EMAX>(lds, r * N2, 1, lp, tl, true, tw);
+  cplx<T> v[EMAX];
+  fft_line_x<T, true, EMAX, false, true>(v, lds, rb, 1, lp, tl, tw);
   double acc[P::NS > 0 ? P::NS : 1];
 #pragma unroll
   for (int i = 0; i < P::NS; ++i) acc[i] = red_identity(P::op(i));
-  for (int n0 = tl; n0 < N2; n0 += kPairs * lp.tpl) {
-    typename P::Rec rec[kPairs];
+  {
+    const int Rl = lp.radix[lp.npass - 1];
+    const int nbl = nbutterflies(N2, Rl, tl, lp.tpl);
 #pragma unroll
-    for (int k = 0; k < kPairs; ++k) {
-      const int n = n0 + k * lp.tpl;
-      if (n < N2) rec[k] = p.fetch(g, row, 2 * n);
-    }
+    for (int e0 = 0; e0 < EMAX; e0 += kPairs) {
+      typename P::Rec rec[kPairs];
 #pragma unroll
-    for (int k = 0; k < kPairs; ++k) {
-      const int n = n0 + k * lp.tpl;
-      if (n < N2) {
-        const cplx<T> z = L(n);
-        p.take(g, row, 2 * n, rec[k], z.x, z.y, acc);
+      for (int k = 0; k < kPairs; ++k) {
+        const int e = e0 + k;
+        if (e / Rl < nbl) rec[k] = p.fetch(g, row, 2 * slot_element(e, Rl, N2, tl, lp.tpl));
+      }
+#pragma unroll
+      for (int k = 0; k < kPairs; ++k) {
+        const int e = e0 + k;
+        if (e / Rl < nbl) p.take(g, row, 2 * slot_element(e, Rl, N2, tl, lp.tpl), rec[k], v[e].x, v[e].y, acc);
       }
     }
   }
@@ -606,7 +642,7 @@ __global__ void __launch_bounds__(1024) k_bisect(Geom g, InstArr<T> ia) {
     for (size_t q = threadIdx.x; q < Mt; q += blockDim.x) {
       const int i = (int)(q / g.ny), j = (int)(q - (size_t)i * g.ny);
       if (in.surf && !in.surf[q]) continue;
-      const double g0 = (double)in.u[(size_t)i * g.My + j] - in.h[q];
+      const double g0 = (double)in.u[(size_t)i * g.My + j] - (double)in.h[q];
       v += (g0 - x < 0.0) ? 1.0 : 0.0;
     }
     int ops[1] = {RSUM};
@@ -967,6 +1003,12 @@ struct ElemEnergy {
   __device__ void finalize(double* a) const { in.sc->fun = -a[0] / 2; }
 };
 
+// FP32 sessions keep their own FP32 copy of the heights.
+__global__ void k_heights_to_f32(const double* __restrict__ h, float* __restrict__ out, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = (float)h[i];
+}
+
 // ---------------------------------------------------------------------------
 // Green's function set-up kernels
 
@@ -1158,6 +1200,57 @@ struct cm_plan {
 
 namespace {
 
+// Launch shape of the FFT passes; 0 = choose automatically.
+struct Shape {
+  int Cb = 0;    // columns per column-pass block
+  int R = 0;     // rows per row-pass block
+  int erow = 0;  // register elements per thread, row transforms (16 or 32)
+  int ecol = 0;  // register elements per thread, column transforms (16 or 32)
+};
+
+template <typename T>
+bool choose_shape(FftExec& ex, const Shape& sh) {
+  Geom& g = ex.g;
+  const int Mx = g.Mx;
+  const size_t cs = sizeof(cplx<T>);
+  constexpr int kMaxBlock = 256;  // FFT kernels are built with __launch_bounds__(256)
+  // 16 register elements per thread when that fits the block, else 32
+  auto plan_len = [&](int L, int e, LinePlan& lp) {
+    if (e) return make_line_plan(L, kMaxBlock, e, lp);
+    return make_line_plan(L, kMaxBlock, 16, lp) || make_line_plan(L, kMaxBlock, 32, lp);
+  };
+  if (!plan_len(g.N2, sh.erow, ex.row) || !plan_len(Mx, sh.ecol, ex.col)) return false;
+  int Cb = 1;
+  if (sh.Cb) {
+    Cb = sh.Cb;
+    if (g.N2 % Cb || ex.col.tpl * Cb > kMaxBlock || (Cb > 1 && lds_bytes<T>((size_t)Mx * Cb) > 65536)) return false;
+  } else {
+    // contiguous slab of Mx*Cb complex values in LDS; aim at 64-byte row segments
+    const int cb_target = (int)(64 / cs) > 0 ? (int)(64 / cs) : 1;
+    while (Cb * 2 <= cb_target && g.N2 % (Cb * 2) == 0 && lds_bytes<T>((size_t)Mx * Cb * 2) <= 65536 &&
+           ex.col.tpl * Cb * 2 <= kMaxBlock)
+      Cb *= 2;
+  }
+  if (lds_bytes<T>((size_t)Mx * Cb) > 150 * 1024) return false;
+  int R = 1;
+  if (sh.R) {
+    R = sh.R;
+    if (Mx % R || ex.row.tpl * R > kMaxBlock || (R > 1 && lds_bytes<T>((size_t)R * g.N2) > 65536)) return false;
+  } else {
+    while (Mx % (R * 2) == 0 && ex.row.tpl * R * 2 <= kMaxBlock && lds_bytes<T>((size_t)R * 2 * g.N2) <= 65536 &&
+           (ex.row.tpl * R < 128 || (size_t)R * Cb * cs < 128))
+      R *= 2;
+  }
+  if (lds_bytes<T>((size_t)R * g.N2) > 150 * 1024) return false;
+  g.Cb = Cb;
+  g.R = R;
+  ex.lds_row = lds_bytes<T>((size_t)R * g.N2);
+  ex.lds_col = lds_bytes<T>((size_t)Mx * Cb);
+  ex.emax_row = ex.row.E <= 16 ? 16 : 32;
+  ex.emax_col = ex.col.E <= 16 ? 16 : 32;
+  return true;
+}
+
 template <typename T>
 int exec_setup(cm_ctx* ctx, FftExec& ex, int nx, int ny, int Mx, int My, size_t& owned) {
   Geom& g = ex.g;
@@ -1167,32 +1260,10 @@ int exec_setup(cm_ctx* ctx, FftExec& ex, int nx, int ny, int Mx, int My, size_t&
   g.My = My;
   g.N2 = My / 2;
   const size_t cs = sizeof(cplx<T>);
-  constexpr int kMaxBlock = 256;  // FFT kernels are built with __launch_bounds__(256)
-  // 16 register elements per thread when that fits the block, else 32
-  auto plan_len = [&](int L, LinePlan& lp) {
-    return make_line_plan(L, kMaxBlock, 16, lp) || make_line_plan(L, kMaxBlock, 32, lp);
-  };
-  if (!plan_len(g.N2, ex.row) || !plan_len(Mx, ex.col))
+  if (!choose_shape<T>(ex, Shape{}))
     return fail(ctx, CM_E_UNSUPPORTED,
                 "grid " + std::to_string(Mx) + "x" + std::to_string(My) +
-                    " not supported: Mx and My/2 must factor into 2, 3, 5");
-  // column block width: contiguous slab of Mx*Cb complex values in LDS
-  int Cb = 1;
-  const int cb_target = (int)(64 / cs) > 0 ? (int)(64 / cs) : 1;  // 64-byte row segments
-  while (Cb * 2 <= cb_target && g.N2 % (Cb * 2) == 0 && (size_t)Mx * Cb * 2 * cs <= 65536 &&
-         ex.col.tpl * Cb * 2 <= kMaxBlock)
-    Cb *= 2;
-  if (lds_bytes<T>((size_t)Mx * Cb) > 150 * 1024) return fail(ctx, CM_E_UNSUPPORTED, "column length exceeds LDS capacity");
-  g.Cb = Cb;
-  int R = 1;
-  while (Mx % (R * 2) == 0 && ex.row.tpl * R * 2 <= kMaxBlock && (size_t)R * 2 * g.N2 * cs <= 65536 &&
-         (ex.row.tpl * R < 128 || (size_t)R * Cb * cs < 128))
-    R *= 2;
-  g.R = R;
-  ex.lds_row = lds_bytes<T>((size_t)R * g.N2);
-  ex.lds_col = lds_bytes<T>((size_t)Mx * Cb);
-  ex.emax_row = ex.row.E <= 16 ? 16 : 32;
-  ex.emax_col = ex.col.E <= 16 ? 16 : 32;
+                    " not supported: Mx and My/2 must factor into 2, 3, 5 and a column must fit in LDS");
   auto twr = host_twiddles<T>(g.N2, g.N2, g.N2);
   auto twc = host_twiddles<T>(Mx, Mx, Mx);
   auto twp = host_twiddles<T>(My, g.N2 / 2 + 1, My);
@@ -1214,7 +1285,7 @@ void allow_big_lds(const void* kern, size_t lds) {
   static std::vector<const void*> done;
   std::lock_guard<std::mutex> lk(mu);
   if (std::find(done.begin(), done.end(), kern) != done.end()) return;
-  (void)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  (void)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max<size_t>(lds, 65536));
   done.push_back(kern);
 }
 
@@ -1399,7 +1470,81 @@ int plan_build(cm_plan* P) {
   if ((rc = dev_alloc(ctx, &P->S, Sb))) return rc;
   P->bytes_owned += Sb;
   const size_t M = (size_t)P->Mx * P->My;
-  P->nblk_elem = (int)std::max<size_t>(1, std::min<size_t>(2048, (M + 2047) / 2048));
+  // element-wise kernels: blocks stride over rows; few enough blocks that the
+  // last block's pass over the partials stays short
+  P->nblk_elem = (int)std::max<size_t>(1, std::min<size_t>(std::min<size_t>(512, (size_t)P->Mx), (M + 2047) / 2048));
+  return 0;
+}
+
+template <typename T>
+void evaluate_ex(cm_plan* P, const FftExec& ex, const void* in, void* out, int useK, hipStream_t st) {
+  InstArr<T> ia{};
+  ia.arr = nullptr;
+  ia.one.src = (const T*)in;
+  ia.one.dst = (T*)out;
+  ia.one.S = (cplx<T>*)P->S;
+  ia.one.G = (const T*)P->G;
+  ia.one.K = (const T*)P->K;
+  launch_r2c<T, ProdCopy<T>>(ex, ia, 1, 0, st);
+  launch_col<T, 0>(ex, ia, 1, useK, 0, st);
+  launch_c2r<T, ConsCopy<T>>(ex, ia, 1, 0, st);
+}
+
+// Time the three passes of evaluate_disp over candidate launch shapes and keep
+// the fastest (columns per block, rows per block, register elements).
+template <typename T>
+int autotune(cm_plan* P, int reps, std::string& desc) {
+  cm_ctx* ctx = P->ctx;
+  hipStream_t st = ctx->stream;
+  const size_t M = (size_t)P->Mx * P->My;
+  void* a = nullptr;
+  void* b = nullptr;
+  int rc;
+  if ((rc = dev_alloc(ctx, &a, M * sizeof(T)))) return rc;
+  if ((rc = dev_alloc(ctx, &b, M * sizeof(T)))) {
+    dev_free(ctx, a, M * sizeof(T));
+    return rc;
+  }
+  HIPCHK(ctx, hipMemsetAsync(a, 0, M * sizeof(T), st));
+  hipEvent_t e0, e1;
+  HIPCHK(ctx, hipEventCreate(&e0));
+  HIPCHK(ctx, hipEventCreate(&e1));
+  FftExec best = P->ex;
+  float best_us = 1e30f;
+  int tried = 0;
+  for (int Cb : {1, 2, 4, 8})
+    for (int R : {1, 2, 4, 8})
+      for (int er : {16, 32})
+        for (int ec : {16, 32}) {
+          FftExec trial = P->ex;
+          if (!choose_shape<T>(trial, Shape{Cb, R, er, ec})) continue;
+          evaluate_ex<T>(P, trial, a, b, 0, st);  // warm-up (first launch of a kernel)
+          HIPCHK(ctx, hipEventRecord(e0, st));
+          for (int k = 0; k < reps; ++k) evaluate_ex<T>(P, trial, a, b, 0, st);
+          HIPCHK(ctx, hipEventRecord(e1, st));
+          HIPCHK(ctx, hipEventSynchronize(e1));
+          if (hipGetLastError() != hipSuccess) continue;  // shape rejected by the runtime
+          float ms = 0;
+          HIPCHK(ctx, hipEventElapsedTime(&ms, e0, e1));
+          const float us = ms * 1000.f / reps;
+          ++tried;
+          if (us < best_us) {
+            best_us = us;
+            best = trial;
+          }
+        }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dev_free(ctx, a, M * sizeof(T));
+  dev_free(ctx, b, M * sizeof(T));
+  P->ex = best;
+  char buf[512];
+  snprintf(buf, sizeof buf,
+           "{\"Cb\": %d, \"R\": %d, \"erow\": %d, \"ecol\": %d, \"row_tpl\": %d, \"col_tpl\": %d, "
+           "\"evaluate_us\": %.3f, \"candidates\": %d}",
+           best.g.Cb, best.g.R, best.row.E <= 16 ? 16 : 32, best.col.E <= 16 ? 16 : 32, best.row.tpl, best.col.tpl,
+           best_us, tried);
+  desc = buf;
   return 0;
 }
 
@@ -1424,8 +1569,8 @@ int evaluate(cm_plan* P, const void* in, void* out, int useK, void* stream) {
 // CCG driver
 
 struct WsLayout {
-  size_t t, S, sc, part, ctr, inst, inst0, total;
-  size_t st_t, st_S, st_sc, st_part, st_ctr;  // per-instance strides
+  size_t t, S, sc, part, ctr, hc, inst, inst0, total;
+  size_t st_t, st_S, st_sc, st_part, st_ctr, st_hc;  // per-instance strides
 };
 
 size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -1440,6 +1585,7 @@ WsLayout ws_layout(const cm_plan* P, int B) {
   L.st_sc = al256(sizeof(Scal));
   L.st_part = al256(nparts * sizeof(double));
   L.st_ctr = al256(8 * sizeof(unsigned));
+  L.st_hc = w == 4 ? al256((size_t)P->d.nx * P->d.ny * 4) : 0;  // FP32 copy of the heights
   const size_t inst_bytes = al256(std::max(sizeof(Inst<double>), sizeof(Inst<float>)) * B);
   size_t o = 0;
   L.t = o; o += L.st_t * B;
@@ -1447,6 +1593,7 @@ WsLayout ws_layout(const cm_plan* P, int B) {
   L.sc = o; o += L.st_sc * B;
   L.part = o; o += L.st_part * B;
   L.ctr = o; o += L.st_ctr * B;
+  L.hc = o; o += L.st_hc * B;
   L.inst = o; o += inst_bytes;
   L.inst0 = o; o += inst_bytes;
   L.total = o;
@@ -1559,7 +1706,15 @@ int session_begin(cm_plan* P, int B, const cm_ccg_params* prm, const double* con
     I.f = (T*)fs[b];
     I.t = (T*)(base + L.t + b * L.st_t);
     I.cm = cms[b];
-    I.h = hs[b];
+    if (std::is_same<T, double>::value) {
+      I.h = (const T*)hs[b];
+    } else {
+      float* hc = (float*)(base + L.hc + b * L.st_hc);
+      const size_t nh = (size_t)P->d.nx * P->d.ny;
+      hipLaunchKernelGGL(k_heights_to_f32, dim3((unsigned)std::min<size_t>(1024, (nh + 255) / 256)), dim3(256), 0,
+                         S->st, hs[b], hc, nh);
+      I.h = (const T*)hc;
+    }
     I.surf = surfs ? surfs[b] : nullptr;
     I.S = (cplx<T>*)(base + L.S + b * L.st_S);
     I.G = (const T*)P->G;
@@ -1898,6 +2053,37 @@ int cm_plan_grid(cm_plan* plan, int* Mx, int* My) {
 int cm_plan_green_host(cm_plan* plan, double* out) {
   if (!plan || !out) return fail(nullptr, CM_E_ARG, "null argument");
   std::memcpy(out, plan->G_host.data(), plan->G_host.size() * sizeof(double));
+  return 0;
+}
+
+int cm_plan_autotune(cm_plan* plan, int reps, char* out, size_t out_len) {
+  if (!plan || reps < 1) return fail(plan ? plan->ctx : nullptr, CM_E_ARG, "bad argument");
+  std::string d;
+  int rc = plan->prec == CM_FP64 ? autotune<double>(plan, reps, d) : autotune<float>(plan, reps, d);
+  if (rc) return rc;
+  if (out && out_len) {
+    std::strncpy(out, d.c_str(), out_len - 1);
+    out[out_len - 1] = 0;
+  }
+  return 0;
+}
+
+int cm_plan_set_shape(cm_plan* plan, int Cb, int R, int erow, int ecol) {
+  if (!plan) return fail(nullptr, CM_E_ARG, "null plan");
+  FftExec trial = plan->ex;
+  const bool ok = plan->prec == CM_FP64 ? choose_shape<double>(trial, Shape{Cb, R, erow, ecol})
+                                        : choose_shape<float>(trial, Shape{Cb, R, erow, ecol});
+  if (!ok) return fail(plan->ctx, CM_E_ARG, "launch shape not valid for this grid");
+  plan->ex = trial;
+  return 0;
+}
+
+int cm_plan_shape(cm_plan* plan, int* Cb, int* R, int* erow, int* ecol) {
+  if (!plan) return fail(nullptr, CM_E_ARG, "null plan");
+  if (Cb) *Cb = plan->ex.g.Cb;
+  if (R) *R = plan->ex.g.R;
+  if (erow) *erow = plan->ex.row.E <= 16 ? 16 : 32;
+  if (ecol) *ecol = plan->ex.col.E <= 16 ? 16 : 32;
   return 0;
 }
 

@@ -119,6 +119,16 @@ int cm_plan_grid(cm_plan* plan, int* Mx, int* My);
  * including the -1/(area_per_pt * Mx * My) factor of evaluate_disp) to host. */
 int cm_plan_green_host(cm_plan* plan, double* host_out);
 
+/* Autotuner: time evaluate_disp over the candidate launch shapes of the FFT
+ * passes (columns per column block, rows per row block, register elements per
+ * thread) and keep the fastest; `out` receives a JSON description.  Must not
+ * run while a session of this plan is in flight.  cm_plan_set_shape applies a
+ * cached choice (0 = automatic), cm_plan_shape reads the current one.  No
+ * reference counterpart (muGrid picks FFTW plans internally). */
+int cm_plan_autotune(cm_plan* plan, int reps, char* out, size_t out_len);
+int cm_plan_set_shape(cm_plan* plan, int Cb, int R, int erow, int ecol);
+int cm_plan_shape(cm_plan* plan, int* Cb, int* R, int* erow, int* ecol);
+
 /* u = evaluate_disp(f) on the (Mx, My) grid (FFTElasticHalfSpace.py:573-589,
  * padded branch of 1141-1164).  f, u: device arrays of the plan precision;
  * in-place (u == f) is allowed. */

@@ -1,0 +1,58 @@
+"""Every launch shape the autotuner can choose computes the same operator.
+
+For a few grids, each valid (columns per block, rows per block, register
+elements row/column) shape is applied to the plan and evaluate_disp /
+evaluate_force are compared with the oracle; then the tuner's pick is cached
+and re-applied.
+"""
+
+import itertools
+import json
+
+import numpy as np
+import pytest
+
+from contactmechanics_amd import FreeFFTElasticHalfSpace, PeriodicFFTElasticHalfSpace
+from contactmechanics_amd import _native
+from contactmechanics_amd.autotune import tune
+from oracle.halfspace import FreeHalfSpace, PeriodicHalfSpace
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("cls,ocls,nx,ny,prec", [
+    (PeriodicFFTElasticHalfSpace, PeriodicHalfSpace, 128, 96, "fp64"),
+    (PeriodicFFTElasticHalfSpace, PeriodicHalfSpace, 256, 256, "fp32"),
+    (FreeFFTElasticHalfSpace, FreeHalfSpace, 48, 80, "fp64"),
+])
+def test_all_shapes_agree(gpu, cls, ocls, nx, ny, prec):
+    g = cls((nx, ny), 1.7, (1.0, 2.0), precision=prec)
+    c = ocls((nx, ny), 1.7, (1.0, 2.0))
+    f = np.random.default_rng(nx + ny).standard_normal(g.nb_domain_grid_pts)
+    ud, fd = c.evaluate_disp(f), c.evaluate_force(f)
+    tol = 1e-9 if prec == "fp64" else 3e-6
+    n_ok = 0
+    for Cb, R, er, ec in itertools.product([1, 2, 4, 8], [1, 2, 4, 8], [16, 32], [16, 32]):
+        try:
+            g.plan.set_shape(Cb, R, er, ec)
+        except _native.CmError:
+            continue
+        n_ok += 1
+        assert np.max(np.abs(g.evaluate_disp(f) - ud)) / np.max(np.abs(ud)) < tol, (Cb, R, er, ec)
+        assert np.max(np.abs(g.evaluate_force(f) - fd)) / np.max(np.abs(fd)) < tol * 10, (Cb, R, er, ec)
+    assert n_ok >= 4
+
+
+def test_tune_and_cache(gpu, tmp_path):
+    cache = str(tmp_path / "tune.json")
+    g = PeriodicFFTElasticHalfSpace((256, 256), 1.0, (1.0, 1.0))
+    d = tune(g, cache_path=cache, reps=3)
+    assert not d["cached"] and d["candidates"] >= 1
+    g2 = PeriodicFFTElasticHalfSpace((256, 256), 1.0, (1.0, 1.0))
+    d2 = tune(g2, cache_path=cache)
+    assert d2["cached"] and g2.plan.shape()["Cb"] == d["Cb"] and g2.plan.shape()["R"] == d["R"]
+    with open(cache) as fh:
+        assert len(json.load(fh)) == 1
+    f = np.random.default_rng(0).standard_normal((256, 256))
+    c = PeriodicHalfSpace((256, 256), 1.0, (1.0, 1.0))
+    assert np.max(np.abs(g2.evaluate_disp(f) - c.evaluate_disp(f))) / np.max(np.abs(c.evaluate_disp(f))) < 1e-12

@@ -190,3 +190,19 @@ def test_system_api(gpu):
     assert sys_.compute_contact_area() == np.sum(res.jac > 0) * sys_.area_per_pt
     gap = sys_.compute_gap(sys_.disp, sys_.offset)
     assert np.min(gap) > -1e-9
+
+
+@pytest.mark.parametrize("bc", ["periodic", "free"])
+def test_mixed_precision_reaches_fp64_accuracy(gpu, bc):
+    # FP32 bulk iterations + FP64 finish: within 1e-6 (L2, relative) of the
+    # FP64 solve at the same tight tolerance; pure FP32 stalls near 1e-5
+    n = 128
+    h = rough(n, 21)
+    cls = PeriodicFFTElasticHalfSpace if bc == "periodic" else FreeFFTElasticHalfSpace
+    kw = dict(external_force=0.01, pentol=1e-11, forcetol=1e-14, thermotol=1e-10, maxiter=20000)
+    r64 = ccg_gpu(cls((n, n), 1.0, (1.0, 1.0), precision="fp64"), h, **kw)
+    rmx = ccg_gpu(cls((n, n), 1.0, (1.0, 1.0), precision="mixed"), h, **kw)
+    assert r64.success and rmx.success
+    assert rmx.nit == rmx.nit_fp32 + rmx.nit_fp64 and rmx.nit_fp32 > 0
+    err = np.linalg.norm(rmx.jac - r64.jac) / np.linalg.norm(r64.jac)
+    assert err < 1e-6, err
