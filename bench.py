@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Constrained-CG benchmark on one MI355X.
 
-Workload (default, --config 2): periodic 2048^2 self-affine surface
+Workload (default, --workload c2): periodic 2048^2 self-affine surface
 (H = 0.8, RMS slope 0.1, seed 2048), mean pressure 0.01 E*, E* = L = 1.
 
 A "step" is one Polonsky-Keer iteration over the whole grid (two FFT
@@ -30,17 +30,20 @@ sys.path.insert(0, ROOT)
 
 def kernel_bytes(name, M, Mt, w):
     """Bytes each fused kernel must read + write once per launch (FP width w;
-    heights are FP64, contact masks one byte)."""
+    heights 8 bytes on the FP64 path and 4 on the FP32 path, contact masks one
+    byte).  The start of an iteration (contact set, gap sums) is fused into
+    the displacement pass, so "phase_a" moves no bytes per iteration."""
+    hw = 8 if w == 8 else 4
     return {
-        "phase_a": M * (2 * w + 1) + 8 * Mt,
+        "phase_a": 0,
         "bisect": 0,
-        "dir_r2c": M * (4 * w + 1) + 8 * Mt,
+        "dir_r2c": M * (4 * w + 1) + hw * Mt,
         "col_conv1": int(2.5 * w * M),
         "tau_c2r": M * (2 * w + 1),
-        "phase_b": M * (3 * w + 1) + 8 * Mt,
-        "force_r2c": M * (5 * w + 2) + 8 * Mt,
+        "phase_b": M * (3 * w + 1) + hw * Mt,
+        "force_r2c": M * (5 * w + 2) + hw * Mt,
         "col_conv2": int(2.5 * w * M),
-        "disp_c2r": M * (3 * w + 1) + 8 * Mt,
+        "disp_c2r": M * (4 * w + 2) + hw * Mt,
     }[name]
 
 
@@ -100,21 +103,135 @@ def cpu_baseline(cfg, iters, substrate_gpu):
     n = cfg["n"]
     hs = PeriodicHalfSpace((n, n), 1.0, (1.0, 1.0))
     kw = dict(external_force=cfg["external_force"], pentol=0.0, forcetol=0.0, thermotol=0.0)
+    # set-up + 1 iteration, then set-up + 1 + iters iterations: the
+    # difference times exactly `iters` iterations
     t0 = time.perf_counter()
-    ccg_cpu(hs, cfg["heights"], maxiter=0, **kw)
-    t_setup = time.perf_counter() - t0
+    ccg_cpu(hs, cfg["heights"], maxiter=1, **kw)
+    t_one = time.perf_counter() - t0
     t0 = time.perf_counter()
-    ref = ccg_cpu(hs, cfg["heights"], maxiter=iters, **kw)
-    t_run = time.perf_counter() - t0 - t_setup
+    ref = ccg_cpu(hs, cfg["heights"], maxiter=iters + 1, **kw)
+    t_run = time.perf_counter() - t0 - t_one
     # same iterations on the GPU (FP64) for a same-seed error figure
     from contactmechanics_amd import constrained_conjugate_gradients as ccg_gpu
-    g = ccg_gpu(substrate_gpu, cfg["heights"], maxiter=iters, **kw)
+    g = ccg_gpu(substrate_gpu, cfg["heights"], maxiter=iters + 1, **kw)
     err = float(np.max(np.abs(g.jac - ref.jac)) / np.max(np.abs(ref.jac)))
     return dict(seconds=t_run, iterations=iters, gpu_vs_oracle_linf=err)
 
 
+def pressure_errors(p, ref, area_per_pt=None):
+    """L2 / Linf relative pressure errors and relative contact-area error."""
+    import numpy as np
+    p = np.asarray(p, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    a, b = int((p > 0).sum()), int((ref > 0).sum())
+    return dict(l2=float(np.linalg.norm(p - ref) / np.linalg.norm(ref)),
+                linf=float(np.max(np.abs(p - ref)) / np.max(np.abs(ref))),
+                area_rel=abs(a - b) / max(b, 1))
+
+
+def oracle_jac(prob):
+    from oracle.ccg import constrained_conjugate_gradients as ccg_cpu
+    from oracle.halfspace import FreeHalfSpace, PeriodicHalfSpace
+    n = prob["n"]
+    cls = PeriodicHalfSpace if prob["bc"] == "periodic" else FreeHalfSpace
+    r = ccg_cpu(cls((n, n), 1.0, (1.0, 1.0)), prob["heights"], external_force=prob["external_force"],
+                hardness=prob.get("hardness"), pentol=prob["pentol"], forcetol=prob["forcetol"],
+                thermotol=prob["thermotol"], maxiter=prob["maxiter"])
+    return r.jac
+
+
+def bench_batch(args):
+    """C4: 256 independent instances of uneven size in one batched run."""
+    import numpy as np
+    import torch
+    from contactmechanics_amd import workloads
+    from contactmechanics_amd.batch import PlanCache, solve_batch
+    recs = workloads.config4(args.count)
+    t0 = time.perf_counter()
+    probs = [workloads.materialize(r, args.gtol) for r in recs]
+    print(f"# c4: {len(probs)} instances prepared in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    out = {}
+    jac = {}
+    for prec in ("fp32", "fp64"):
+        cache = PlanCache(prec)
+        for n, bc in sorted({(p["n"], p["bc"]) for p in probs}):
+            cache.get(n, bc)
+        solve_batch(probs[:4], precision=prec, cache=cache)  # warm-up
+        res, info = solve_batch(probs, precision=prec, cache=cache)
+        ok = sum(r["stats"]["status"] == 0 for r in res)
+        out[prec] = dict(seconds=info["seconds"], grid_pt_iters=info["grid_pt_iters"], groups=info["groups"],
+                         value=info["grid_pt_iters"] / info["seconds"], instances_per_s=len(probs) / info["seconds"],
+                         converged=ok, iterations=int(sum(r["stats"]["nit"] for r in res)))
+        jac[prec] = [r["jac"].cpu().numpy() for r in res]
+        print(f"# c4 {prec}: {out[prec]}", file=sys.stderr, flush=True)
+        del res
+        torch.cuda.synchronize()
+    e = [pressure_errors(a, b) for a, b in zip(jac["fp32"], jac["fp64"])]
+    errs = dict(fp32_vs_fp64_gpu={k: max(x[k] for x in e) for k in e[0]})
+    small = [i for i, p in enumerate(probs) if p["n"] == 128][:2]
+    if small and not args.no_cpu:
+        eo = [pressure_errors(jac["fp64"][i], oracle_jac(probs[i])) for i in small]
+        errs["fp64_gpu_vs_oracle_128"] = {k: max(x[k] for x in eo) for k in eo[0]}
+    line = {
+        "metric": "C4 batched uneven solves, grid-pt-iters/s (FP32 fields, FP64 reductions)",
+        "value": out["fp32"]["value"], "unit": "grid-pt-iters/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"C4 {len(probs)} instances, sides {{128..1024}}, 25% free, gtol {args.gtol}"},
+        "fp32": out["fp32"], "fp64": out["fp64"], "errors": errs,
+    }
+    print(json.dumps(line))
+
+
+def bench_serve(args):
+    """C5: Poisson stream of mixed-size requests through the request server."""
+    import numpy as np
+    import torch
+    from contactmechanics_amd import workloads
+    from contactmechanics_amd.batch import solve_batch
+    from contactmechanics_amd.serve import ContactServer
+    recs = workloads.config5(args.count, args.rate, side_max=args.side_max)
+    t0 = time.perf_counter()
+    reqs = [workloads.materialize(r, args.gtol) for r in recs]
+    print(f"# c5: {len(reqs)} requests prepared in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    srv = ContactServer(precision="fp32", streams=4)
+    for n, bc in sorted({(r["n"], r["bc"]) for r in reqs}):
+        srv.cache.get(n, bc)
+    got = {}
+
+    def keep(rec, s):
+        if rec["n"] <= args.check_side:
+            got[rec["index"]] = s.pressures(0).cpu().numpy().copy()
+
+    done, summary = srv.run(reqs, on_done=keep)
+    print(f"# c5 summary: {summary}", file=sys.stderr, flush=True)
+    chk = [r for r in reqs if r["index"] in got]
+    errs = None
+    if chk:
+        res, _ = solve_batch(chk, precision="fp64")
+        e = [pressure_errors(got[r["index"]], x["jac"].cpu().numpy()) for r, x in zip(chk, res)]
+        errs = dict(requests_checked=len(chk), fp32_vs_fp64_gpu={k: max(x[k] for x in e) for k in e[0]})
+    lat = sorted(d["latency"] for d in done)
+    line = {
+        "metric": "C5 served request stream: p50/p99 solve latency at stated throughput",
+        "value": summary["p99_latency_s"], "unit": "s (p99 latency)", "n_gpus": 1,
+        "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"C5 {len(reqs)} Poisson requests at {args.rate}/s, sides 256..{args.side_max}, "
+                               f"half free, 20% hardness, gtol {args.gtol}"},
+        "summary": summary, "statuses": [d["status"] for d in done], "errors": errs,
+        "max_latency_s": lat[-1] if lat else None,
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5"],
+                    help="c2 (default): single 2048^2 solve; c4: batched uneven solves; c5: served stream")
+    ap.add_argument("--count", type=int, default=256, help="c4 instances / c5 requests")
+    ap.add_argument("--rate", type=float, default=2.0, help="c5 arrival rate (requests/s)")
+    ap.add_argument("--side-max", type=int, default=4096, help="c5 largest grid side")
+    ap.add_argument("--check-side", type=int, default=512, help="c5: re-solve requests up to this side in FP64")
+    ap.add_argument("--gtol", type=float, default=1e-6)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
@@ -135,6 +252,10 @@ def main():
     from contactmechanics_amd import PeriodicFFTElasticHalfSpace, workloads
 
     torch.cuda.set_device(0)
+    if args.workload == "c4":
+        return bench_batch(args)
+    if args.workload == "c5":
+        return bench_serve(args)
     if args.calib_copy:
         src = torch.ones(32 * 2 ** 20, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
@@ -153,6 +274,7 @@ def main():
     print(f"# fp64: {tp64['iterations']} its in {tp64['seconds']:.4f}s", file=sys.stderr, flush=True)
     tp32 = throughput(sub32, cfg["heights"], cfg["external_force"], args.steps, args.warmup, "fp32")
     print(f"# fp32: {tp32['iterations']} its in {tp32['seconds']:.4f}s", file=sys.stderr, flush=True)
+    print(f"# kernel us fp64 {tp64['kernel_us']}\n# kernel us fp32 {tp32['kernel_us']}", file=sys.stderr, flush=True)
     assert tp64["iterations"] == args.steps and tp32["iterations"] == args.steps, "iterations were skipped"
     value = M * args.steps / tp64["seconds"]
     value32 = M * args.steps / tp32["seconds"]

@@ -22,8 +22,12 @@ def _key(substrate):
 
 
 def tune(substrate, cache_path=DEFAULT_CACHE, reps=10, force=False):
-    """Set the fastest launch shape on ``substrate.plan``; returns its description."""
+    """Set the fastest launch shape on ``substrate.plan``; returns its description.
+    FP32 plans keep their automatic shape (the library offers explicit shapes
+    for FP64 plans only)."""
     plan = substrate.plan
+    if substrate.precision != "fp64":
+        return dict(plan.shape(), cached=False, tuned=False)
     key = _key(substrate)
     cache = {}
     if cache_path and os.path.exists(cache_path):

@@ -153,14 +153,16 @@ template <typename T, bool INV> struct Dft<T, 16, INV> {
 };
 
 // LDS keeps complex values at padded positions: one slot is skipped every
-// 128 bytes, so the stride-R Stockham writes of the early passes spread over
-// all banks instead of piling onto a few.
-template <typename T> struct LdsPad {
-  static constexpr int P = sizeof(cplx<T>) == 16 ? 3 : 4;
-};
-template <typename T> __device__ __forceinline__ int lpad(int i) { return i + (i >> LdsPad<T>::P); }
-template <typename T> inline size_t lds_bytes(size_t n) {
-  return (n + (n >> LdsPad<T>::P) + 1) * sizeof(cplx<T>);
+// 128 bytes (shift kPadShift<T>), so the stride-R Stockham writes of the early
+// passes spread over all banks instead of piling onto a few.  A line whose
+// padded slab would not fit 64 KiB runs unpadded (shift kNoPad).
+template <typename T> constexpr int kPadShift = sizeof(cplx<T>) == 16 ? 3 : 4;
+constexpr int kNoPad = 30;
+// (the explicit unpadded branch also keeps the gfx950 backend away from a
+// register-class error it hits with a plain variable shift here)
+__device__ __forceinline__ int lpad(int i, int psh) { return psh == kNoPad ? i : i + (i >> psh); }
+template <typename T> inline size_t lds_bytes(size_t n, int psh) {
+  return (n + (n >> psh)) * sizeof(cplx<T>);
 }
 
 // ---------------------------------------------------------------------------
@@ -170,6 +172,7 @@ struct LinePlan {
   int tpl;        // threads per line
   int E;          // most elements a thread holds in any pass
   int npass;      // number of radix passes
+  int psh;        // LDS padding shift (kPadShift or kNoPad)
   int radix[14];  // radix of each pass (product == L)
 };
 
@@ -178,7 +181,7 @@ struct LinePlan {
 // Line element n lives at lds[base + n*stride].
 template <typename T, int R, bool INV, int EMAX>
 __device__ __forceinline__ void stockham_pass(cplx<T>* lds, int base, int stride, int L, int Ns,
-                                              int tl, int tpl, int nb, const cplx<T>* __restrict__ tw) {
+                                              int tl, int tpl, int nb, const cplx<T>* __restrict__ tw, int psh) {
   constexpr int NBMAX = EMAX / R > 0 ? EMAX / R : 1;
   cplx<T> v[NBMAX][R];
   const int LR = L / R;
@@ -187,7 +190,7 @@ __device__ __forceinline__ void stockham_pass(cplx<T>* lds, int base, int stride
     if (b < nb) {
       const int j = tl + b * tpl;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = lds[lpad<T>(base + (j + r * LR) * stride)];
+      for (int r = 0; r < R; ++r) v[b][r] = lds[lpad(base + (j + r * LR) * stride, psh)];
     }
   }
   __syncthreads();
@@ -207,10 +210,27 @@ __device__ __forceinline__ void stockham_pass(cplx<T>* lds, int base, int stride
       Dft<T, R, INV>::run(v[b]);
       const int o = (j / Ns) * Ns * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[lpad<T>(base + (o + r * Ns) * stride)] = v[b][r];
+      for (int r = 0; r < R; ++r) lds[lpad(base + (o + r * Ns) * stride, psh)] = v[b][r];
     }
   }
   __syncthreads();
+}
+
+// One Stockham pass of radix R (R <= EMAX; radix 16 only exists in kernels
+// built for 16 or more register elements, so smaller builds stay lean).
+template <typename T, bool INV, int EMAX>
+__device__ __forceinline__ void pass_dispatch(int R, cplx<T>* lds, int base, int stride, int L, int Ns, int tl,
+                                              int tpl, int nb, const cplx<T>* __restrict__ tw, int psh) {
+  switch (R) {
+    case 2: stockham_pass<T, 2, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw, psh); break;
+    case 3: stockham_pass<T, 3, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw, psh); break;
+    case 4: stockham_pass<T, 4, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw, psh); break;
+    case 5: stockham_pass<T, 5, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw, psh); break;
+    case 16:
+      if constexpr (EMAX >= 16) stockham_pass<T, 16, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw, psh);
+      break;
+    default: stockham_pass<T, 8, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw, psh); break;
+  }
 }
 
 // Full in-place LDS FFT of one line (all threads of the block must call it;
@@ -223,14 +243,7 @@ __device__ void fft_line(cplx<T>* lds, int base, int stride, const LinePlan& p, 
     const int R = p.radix[s];
     const int nbf = p.L / R;  // butterflies of this pass, dealt round-robin to threads
     const int nb = (active && tl < nbf) ? (nbf - tl + p.tpl - 1) / p.tpl : 0;
-    switch (R) {
-      case 2: stockham_pass<T, 2, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
-      case 3: stockham_pass<T, 3, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
-      case 4: stockham_pass<T, 4, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
-      case 5: stockham_pass<T, 5, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
-      case 16: stockham_pass<T, 16, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
-      default: stockham_pass<T, 8, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
-    }
+    pass_dispatch<T, INV, EMAX>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw, p.psh);
     Ns *= R;
   }
 }
@@ -254,7 +267,7 @@ __device__ __forceinline__ int slot_element(int e, int R, int L, int tl, int tpl
 
 template <typename T, int R, bool INV, int EMAX>
 __device__ __forceinline__ void first_pass_regs(const cplx<T>* v, cplx<T>* lds, int base, int stride, int L,
-                                                int tl, int tpl) {
+                                                int tl, int tpl, int psh) {
   constexpr int NBMAX = EMAX / R > 0 ? EMAX / R : 1;
   const int nb = nbutterflies(L, R, tl, tpl);
 #pragma unroll
@@ -266,14 +279,14 @@ __device__ __forceinline__ void first_pass_regs(const cplx<T>* v, cplx<T>* lds, 
       Dft<T, R, INV>::run(w);
       const int o = (tl + b * tpl) * R;  // Ns = 1
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[lpad<T>(base + (o + r) * stride)] = w[r];
+      for (int r = 0; r < R; ++r) lds[lpad(base + (o + r) * stride, psh)] = w[r];
     }
   }
 }
 
 template <typename T, int R, bool INV, int EMAX>
 __device__ __forceinline__ void last_pass_regs(cplx<T>* v, const cplx<T>* lds, int base, int stride, int L,
-                                               int tl, int tpl, const cplx<T>* __restrict__ tw) {
+                                               int tl, int tpl, const cplx<T>* __restrict__ tw, int psh) {
   constexpr int NBMAX = EMAX / R > 0 ? EMAX / R : 1;
   const int nb = nbutterflies(L, R, tl, tpl);
   const int LR = L / R;  // = Ns of the last pass
@@ -283,7 +296,7 @@ __device__ __forceinline__ void last_pass_regs(cplx<T>* v, const cplx<T>* lds, i
       const int j = tl + b * tpl;
       cplx<T> w[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) w[r] = lds[lpad<T>(base + (j + r * LR) * stride)];
+      for (int r = 0; r < R; ++r) w[r] = lds[lpad(base + (j + r * LR) * stride, psh)];
       if (LR > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
@@ -308,12 +321,14 @@ __device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const
   int s0 = 0, s1 = p.npass, Ns = 1;
   if (IN_REGS) {
     switch (p.radix[0]) {
-      case 2: first_pass_regs<T, 2, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
-      case 3: first_pass_regs<T, 3, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
-      case 4: first_pass_regs<T, 4, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
-      case 5: first_pass_regs<T, 5, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
-      case 16: first_pass_regs<T, 16, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
-      default: first_pass_regs<T, 8, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
+      case 2: first_pass_regs<T, 2, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, p.psh); break;
+      case 3: first_pass_regs<T, 3, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, p.psh); break;
+      case 4: first_pass_regs<T, 4, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, p.psh); break;
+      case 5: first_pass_regs<T, 5, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, p.psh); break;
+      case 16:
+        if constexpr (EMAX >= 16) first_pass_regs<T, 16, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, p.psh);
+        break;
+      default: first_pass_regs<T, 8, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, p.psh); break;
     }
     __syncthreads();
     Ns = p.radix[0];
@@ -323,14 +338,7 @@ __device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const
   for (int s = s0; s < s1; ++s) {
     const int R = p.radix[s];
     const int nb = nbutterflies(p.L, R, tl, p.tpl);
-    switch (R) {
-      case 2: stockham_pass<T, 2, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
-      case 3: stockham_pass<T, 3, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
-      case 4: stockham_pass<T, 4, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
-      case 5: stockham_pass<T, 5, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
-      case 16: stockham_pass<T, 16, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
-      default: stockham_pass<T, 8, INV, EMAX>(lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw); break;
-    }
+    pass_dispatch<T, INV, EMAX>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw, p.psh);
     Ns *= R;
   }
   if (OUT_REGS) {
@@ -340,16 +348,18 @@ __device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const
       const int nb = nbutterflies(p.L, R, tl, p.tpl);
 #pragma unroll
       for (int e = 0; e < EMAX; ++e)
-        if (e / R < nb) v[e] = lds[lpad<T>(base + slot_element(e, R, p.L, tl, p.tpl) * stride)];
+        if (e / R < nb) v[e] = lds[lpad(base + slot_element(e, R, p.L, tl, p.tpl) * stride, p.psh)];
       return;
     }
     switch (p.radix[p.npass - 1]) {
-      case 2: last_pass_regs<T, 2, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
-      case 3: last_pass_regs<T, 3, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
-      case 4: last_pass_regs<T, 4, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
-      case 5: last_pass_regs<T, 5, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
-      case 16: last_pass_regs<T, 16, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
-      default: last_pass_regs<T, 8, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
+      case 2: last_pass_regs<T, 2, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw, p.psh); break;
+      case 3: last_pass_regs<T, 3, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw, p.psh); break;
+      case 4: last_pass_regs<T, 4, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw, p.psh); break;
+      case 5: last_pass_regs<T, 5, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw, p.psh); break;
+      case 16:
+        if constexpr (EMAX >= 16) last_pass_regs<T, 16, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw, p.psh);
+        break;
+      default: last_pass_regs<T, 8, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw, p.psh); break;
     }
   }
 }
@@ -368,9 +378,17 @@ __device__ __forceinline__ double red_apply(int op, double a, double b) {
   return op == RMAX ? (a > b ? a : b) : (a < b ? a : b);
 }
 
+// Butterfly reduction over the lanes of one wavefront.  A block whose size is
+// not a multiple of 64 has a partial last wave: lanes past the block end do
+// not exist, so their (undefined) values are skipped.
 __device__ __forceinline__ double wave_reduce(int op, double v) {
+  const int lane = threadIdx.x & 63;
+  const int nlanes = min(64, (int)blockDim.x - (int)(threadIdx.x & ~63u));
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = red_apply(op, v, __shfl_xor(v, off, 64));
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(v, off, 64);
+    if ((lane ^ off) < nlanes) v = red_apply(op, v, o);
+  }
   return v;
 }
 

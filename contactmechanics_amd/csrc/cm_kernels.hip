@@ -95,9 +95,23 @@ __device__ __forceinline__ bool is_comp(const Geom& g, const uint8_t* surf, int 
 }
 
 // ---------------------------------------------------------------------------
+// Wait until this wave's outstanding memory operations (here: the
+// write-through partial stores, issued as flat stores, which count in both
+// counters) have completed.
+__device__ __forceinline__ void wait_stores_done() { asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory"); }
+// Agent-scope acquire: invalidates this CU's view of L2 so later plain loads
+// see data written through by other XCDs.
+__device__ __forceinline__ void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+
 // Cross-block reduction: every block reduces NS slots, writes its partials,
 // takes a ticket; the block drawing the last ticket reduces all partials in a
 // fixed order (deterministic result) and returns true.
+//
+// No device-scope release fence per block: on gfx950 such a fence writes back
+// and invalidates the XCD's whole L2, once per block.  Instead the partials are
+// stored with agent-scope (L2 write-through) atomic stores, the wave waits for
+// them to complete (s_waitcnt vmcnt(0)) before it takes its ticket, and only
+// the last block invalidates its L2 (agent-scope acquire) before reading them.
 template <class P>
 __device__ bool reduce_blocks(double* acc, double* part, unsigned* ctr, int nblocks, int bid) {
   constexpr int NS = P::NS;
@@ -109,14 +123,15 @@ __device__ bool reduce_blocks(double* acc, double* part, unsigned* ctr, int nblo
   block_reduce<NS>(acc, ops, scr);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int i = 0; i < NS; ++i) part[(size_t)bid * NS + i] = acc[i];
-    __threadfence();
+    for (int i = 0; i < NS; ++i)
+      __hip_atomic_store(&part[(size_t)bid * NS + i], acc[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    wait_stores_done();
     unsigned tk = atomicAdd(ctr, 1u);
     s_last = (tk == (unsigned)(nblocks - 1));
   }
   __syncthreads();
   if (!s_last) return false;
-  __threadfence();
+  acquire_agent();
   double v[NS > 0 ? NS : 1];
 #pragma unroll
   for (int i = 0; i < NS; ++i) v[i] = red_identity(ops[i]);
@@ -209,7 +224,7 @@ __global__ void __launch_bounds__(256) k_r2c(Geom g, LinePlan lp, const cplx<T>*
 #pragma unroll
   for (int i = 0; i < P::NS; ++i) acc[i] = red_identity(P::op(i));
   const int rb = r * N2;
-  auto L = [&](int n) -> cplx<T>& { return lds[lpad<T>(rb + n)]; };
+  auto L = [&](int n) -> cplx<T>& { return lds[lpad(rb + n, lp.psh)]; };
   // the producer feeds the first FFT pass directly (first-pass order)
   cplx<T> v[EMAX];
   {
@@ -258,7 +273,7 @@ __global__ void __launch_bounds__(256) k_r2c(Geom g, LinePlan lp, const cplx<T>*
   const int tot = g.R * N2;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int kk = i % g.Cb, rr = (i / g.Cb) % g.R, cb = i / RC;
-    S[((size_t)cb * g.Mx + blockIdx.x * g.R + rr) * g.Cb + kk] = lds[lpad<T>(rr * N2 + cb * g.Cb + kk)];
+    S[((size_t)cb * g.Mx + blockIdx.x * g.R + rr) * g.Cb + kk] = lds[lpad(rr * N2 + cb * g.Cb + kk, lp.psh)];
   }
   if constexpr (P::NS > 0) {
     if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
@@ -269,7 +284,7 @@ __global__ void __launch_bounds__(256) k_r2c(Geom g, LinePlan lp, const cplx<T>*
 // Column pass.  MODE 0: forward * G * inverse; 1: forward only; 2: inverse only.
 // skip: 0 never, 1 when the solve is done, 2 when done or in a mixing step.
 template <typename T, int EMAX, int MODE>
-__global__ void __launch_bounds__(256) k_col(Geom g, LinePlan lp, const cplx<T>* __restrict__ tw,
+__global__ void __launch_bounds__(256) k_col(Geom g, LinePlan lp, LinePlan rv, const cplx<T>* __restrict__ tw,
                                               InstArr<T> ia, int useK, int skip) {
   extern __shared__ __align__(16) unsigned char smem_col[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_col);
@@ -301,15 +316,15 @@ __global__ void __launch_bounds__(256) k_col(Geom g, LinePlan lp, const cplx<T>*
 #pragma unroll
     for (int e = 0; e < EMAX; ++e)
       if (e / Rl < nbl) v[e] = v[e] * Gk[slot_element(e, Rl, Mx, tl, lp.tpl)];
-    LinePlan rv = lp;  // inverse radices reversed: its first pass takes the slots as they are
-    for (int q = 0; q < lp.npass; ++q) rv.radix[q] = lp.radix[lp.npass - 1 - q];
+    // rv: the plan with its radices reversed, so the inverse transform's first
+    // pass takes the forward transform's last-pass register slots as they are
     fft_line_x<T, true, EMAX, true, true>(v, lds, c, Cb, rv, tl, tw);
 #pragma unroll
     for (int e = 0; e < EMAX; ++e)
       if (e / R0 < nb0) S[base + (size_t)slot_element(e, R0, Mx, tl, lp.tpl) * Cb + c] = v[e];
     return;
   }
-  for (int i = threadIdx.x; i < n; i += blockDim.x) lds[lpad<T>(i)] = S[base + i];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) lds[lpad(i, lp.psh)] = S[base + i];
   __syncthreads();
   const int c = threadIdx.x % Cb, tl = threadIdx.x / Cb;
   if (MODE != 2) fft_line<T, false, EMAX>(lds, c, Cb, lp, tl, true, tw);
@@ -318,7 +333,7 @@ __global__ void __launch_bounds__(256) k_col(Geom g, LinePlan lp, const cplx<T>*
     if (ky != 0) {
       const T* Gk = G + (size_t)ky * Mx;
       for (int x = tl; x < Mx; x += lp.tpl) {
-        cplx<T>& e = lds[lpad<T>(x * Cb + c)];
+        cplx<T>& e = lds[lpad(x * Cb + c, lp.psh)];
         e = e * Gk[x];
       }
     } else {
@@ -328,18 +343,18 @@ __global__ void __launch_bounds__(256) k_col(Geom g, LinePlan lp, const cplx<T>*
       const T* GN = G + (size_t)g.N2 * Mx;
       for (int x = tl; x <= Mx / 2; x += lp.tpl) {
         const int xm = (Mx - x) % Mx;
-        cplx<T> q = lds[lpad<T>(x * Cb)], qm = lds[lpad<T>(xm * Cb)];
+        cplx<T> q = lds[lpad(x * Cb, lp.psh)], qm = lds[lpad(xm * Cb, lp.psh)];
         const T a0 = G0[x], aN = GN[x], b0 = G0[xm], bN = GN[xm];
         cplx<T> nq = q * (T(0.5) * (a0 + aN)) + conj(qm) * (T(0.5) * (a0 - aN));
         cplx<T> nqm = qm * (T(0.5) * (b0 + bN)) + conj(q) * (T(0.5) * (b0 - bN));
-        lds[lpad<T>(x * Cb)] = nq;
-        if (xm != x) lds[lpad<T>(xm * Cb)] = nqm;
+        lds[lpad(x * Cb, lp.psh)] = nq;
+        if (xm != x) lds[lpad(xm * Cb, lp.psh)] = nqm;
       }
     }
     __syncthreads();
   }
   if (MODE != 1) fft_line<T, true, EMAX>(lds, c, Cb, lp, tl, true, tw);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) S[base + i] = lds[lpad<T>(i)];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) S[base + i] = lds[lpad(i, lp.psh)];
 }
 
 // Row pass: packed half spectrum -> real rows (inverse, unnormalised; the
@@ -358,13 +373,13 @@ __global__ void __launch_bounds__(256) k_c2r(Geom g, LinePlan lp, const cplx<T>*
   const int tot = g.R * N2;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int kk = i % g.Cb, rr = (i / g.Cb) % g.R, cb = i / RC;
-    lds[lpad<T>(rr * N2 + cb * g.Cb + kk)] = S[((size_t)cb * g.Mx + blockIdx.x * g.R + rr) * g.Cb + kk];
+    lds[lpad(rr * N2 + cb * g.Cb + kk, lp.psh)] = S[((size_t)cb * g.Mx + blockIdx.x * g.R + rr) * g.Cb + kk];
   }
   __syncthreads();
   const int r = threadIdx.x / lp.tpl, tl = threadIdx.x - r * lp.tpl;
   const int row = blockIdx.x * g.R + r;
   const int rb = r * N2;
-  auto L = [&](int n) -> cplx<T>& { return lds[lpad<T>(rb + n)]; };
+  auto L = [&](int n) -> cplx<T>& { return lds[lpad(rb + n, lp.psh)]; };
   for (int k = tl; k <= N2 / 2; k += lp.tpl) {
     if (k == 0) {
       cplx<T> z = L(0);  // (X[0], X[N2])
@@ -560,8 +575,61 @@ struct ElemZeroPad {
 // ---------------------------------------------------------------------------
 // CCG iteration phases
 
-// Phase A: contact set and gap statistics (:238-273, :312-318 brackets).
-// c = f<0 [& f>-H]; A_contact, A_cg; shifted gap sums for offset and G.
+// Start of an iteration (:235-273, :300-309): counts, PK/mixing switch,
+// offset and G from the shifted gap sums a = {A_contact, A_cg, Sd, Sdd, gmax,
+// gmin} accumulated with shift s->shift.
+__device__ void phase_a_logic(Scal* s, const double* a) {
+  s->A_contact = a[0];
+  s->A_cg = a[1];
+  s->gmax = a[4];
+  s->gmin = a[5];
+  const bool pk = !s->has_H || (a[0] > 0 && a[1] / a[0] > 0.01 && s->mixsteps <= 0);  // :260
+  if (pk) {
+    s->mode = MODE_PK;
+    if (s->delta_str == DS_MIX) s->delta_str = DS_SD;  // :261-262
+    if (s->load_ctl) s->offset = a[1] > 0 ? s->shift + a[2] / a[1] : 0.0;  // :266-269
+    const double m = s->offset - s->shift;
+    double G = a[3] - 2.0 * m * a[2] + a[1] * m * m;  // sum_c (g - offset)^2 (:273)
+    s->G = G < 0.0 ? 0.0 : G;  // clamp round-off below zero, keep NaN
+    s->use_beta = (s->delta > 0 && s->G_old > 0) ? 1 : 0;  // :276
+    s->beta = s->use_beta ? s->delta * (s->G / s->G_old) : 0.0;
+  } else {
+    s->mode = MODE_MIX;
+    if (s->delta_str != DS_MIX) {  // :307-309
+      s->mixsteps = s->minmixsteps;
+      s->delta_str = DS_MIX;
+    }
+  }
+}
+
+// Per-point part of the start of an iteration: contact set from the current
+// forces (:239-249) and the gap sums shifted by `shift` (the offset of the
+// previous iteration, which keeps sum (g - offset)^2 free of cancellation);
+// returns c.
+__device__ __forceinline__ bool phase_a_point(const Scal* s, double shift, double fv, double uv, double hv,
+                                              bool comp, double* acc) {
+  const bool c0 = fv < 0.0;
+  const bool c = c0 && (!s->has_H || fv > -s->H);
+  acc[0] += c0 ? 1.0 : 0.0;
+  acc[1] += c ? 1.0 : 0.0;
+  if (comp) {
+    const double g0 = uv - hv;
+    const double d = g0 - shift;
+    if (c) acc[2] += d;                  // sum over the contact set only (:269)
+    acc[3] += (c ? 1.0 : 0.0) * d * d;   // c * g * g: a NaN gap poisons G as in :273
+    acc[4] = red_apply(RMAX, acc[4], g0);
+    acc[5] = red_apply(RMIN, acc[5], g0);
+  }
+  return c;
+}
+
+// Contact-mask bits: bit 0 = contact set c of the last finished iteration
+// (result.active_set), bit 1 = gap < 0 at the start of that iteration
+// (result.plastic), bit 2 = contact set of the iteration in progress.
+constexpr uint8_t kCur = 1, kGneg = 2, kNext = 4;
+
+// Phase A of the first iteration (later iterations fuse it into ConsDisp):
+// contact set and gap statistics (:238-273, :312-318 brackets).
 template <typename T>
 struct ElemPhaseA {
   static constexpr int NS = 6;  // A_contact, A_cg, Sd, Sdd, gmax, gmin
@@ -577,52 +645,13 @@ struct ElemPhaseA {
     const size_t p = (size_t)row * g.My + col;
     return Rec{ld2(in.f + p), ld2(in.u + p), ldh(g, in.h, in.surf, row, col)};
   }
-  __device__ bool one(const Scal* s, double fv, double uv, double hv, bool comp, double* acc) const {
-    const bool c0 = fv < 0.0;
-    const bool c = c0 && (!s->has_H || fv > -s->H);
-    acc[0] += c0 ? 1.0 : 0.0;
-    acc[1] += c ? 1.0 : 0.0;
-    if (comp) {
-      const double g0 = uv - hv;
-      const double d = g0 - s->shift;
-      if (c) acc[2] += d;                  // sum over the contact set only (:269)
-      acc[3] += (c ? 1.0 : 0.0) * d * d;   // c * g * g: a NaN gap poisons G as in :273
-      acc[4] = red_apply(RMAX, acc[4], g0);
-      acc[5] = red_apply(RMIN, acc[5], g0);
-    }
-    return c;
-  }
   __device__ void emit(const Geom& g, int row, int col, const Rec& r, double* acc) const {
     const Scal* s = in.sc;
-    const bool c0 = one(s, (double)r.f.a, (double)r.u.a, r.h.h0, r.h.c0, acc);
-    const bool c1 = one(s, (double)r.f.b, (double)r.u.b, r.h.h1, r.h.c1, acc);
-    stm(in.cm + (size_t)row * g.My + col, c0 ? 1 : 0, c1 ? 1 : 0);
+    const bool c0 = phase_a_point(s, s->shift, (double)r.f.a, (double)r.u.a, r.h.h0, r.h.c0, acc);
+    const bool c1 = phase_a_point(s, s->shift, (double)r.f.b, (double)r.u.b, r.h.h1, r.h.c1, acc);
+    stm(in.cm + (size_t)row * g.My + col, c0 ? kNext : 0, c1 ? kNext : 0);
   }
-  __device__ void finalize(double* a) const {
-    Scal* s = in.sc;
-    s->it += 1;  // iteration number of this pass (:235-236)
-    s->A_contact = a[0];
-    s->A_cg = a[1];
-    s->gmax = a[4];
-    s->gmin = a[5];
-    const bool pk = !s->has_H || (a[0] > 0 && a[1] / a[0] > 0.01 && s->mixsteps <= 0);  // :260
-    if (pk) {
-      s->mode = MODE_PK;
-      if (s->delta_str == DS_MIX) s->delta_str = DS_SD;  // :261-262
-      if (s->load_ctl) s->offset = a[1] > 0 ? s->shift + a[2] / a[1] : 0.0;  // :266-269
-      const double m = s->offset - s->shift;
-      double G = a[3] - 2.0 * m * a[2] + a[1] * m * m;  // sum_c (g - offset)^2 (:273)
-      s->G = G < 0.0 ? 0.0 : G;  // clamp round-off below zero, keep NaN
-      s->use_beta = (s->delta > 0 && s->G_old > 0) ? 1 : 0;  // :276
-      s->beta = s->use_beta ? s->delta * (s->G / s->G_old) : 0.0;
-    } else {
-      s->mode = MODE_MIX;
-      if (s->delta_str != DS_MIX) {  // :307-309
-        s->mixsteps = s->minmixsteps;
-        s->delta_str = DS_MIX;
-      }
-    }
-  }
+  __device__ void finalize(double* a) const { phase_a_logic(in.sc, a); }
 };
 
 // scipy.optimize.bisect (xtol=2e-12, rtol=4*eps, maxiter=100) of
@@ -683,6 +712,7 @@ __global__ void __launch_bounds__(1024) k_bisect(Geom g, InstArr<T> ia) {
     if (status) {
       s->status = status;
       s->done = 1;
+      s->it += 1;  // the failed iteration counts
     } else {
       s->offset = res;
     }
@@ -718,8 +748,8 @@ struct ProdDir {
   }
   __device__ void make(const Geom& g, int row, int col, const Rec& r, T& a, T& b, double* acc) const {
     const Scal* s = in.sc;
-    a = one(s, r.h.c0, r.cm.a & 1, (double)r.u.a, r.h.h0, r.t.a, acc);
-    b = one(s, r.h.c1, r.cm.b & 1, (double)r.u.b, r.h.h1, r.t.b, acc);
+    a = one(s, r.h.c0, r.cm.a & kNext, (double)r.u.a, r.h.h0, r.t.a, acc);
+    b = one(s, r.h.c1, r.cm.b & kNext, (double)r.u.b, r.h.h1, r.t.b, acc);
     st2(in.t + (size_t)row * g.My + col, a, b);
   }
   __device__ void finalize(double* a) const { in.sc->Scgt = a[0]; }
@@ -743,8 +773,8 @@ struct ConsTau {
     return Rec{ld2(in.t + p), ldm(in.cm + p)};
   }
   __device__ void take(const Geom&, int, int, const Rec& r, T a, T b, double* acc) const {
-    if (r.cm.a & 1) acc[0] += (double)a * (double)r.t.a;
-    if (r.cm.b & 1) acc[0] += (double)b * (double)r.t.b;
+    if (r.cm.a & kNext) acc[0] += (double)a * (double)r.t.a;
+    if (r.cm.b & kNext) acc[0] += (double)b * (double)r.t.b;
   }
   __device__ void finalize(double* a) const {
     Scal* s = in.sc;
@@ -790,7 +820,7 @@ struct ElemPhaseB {
                          double* acc) const {
     double gap = 0.0;
     if (comp) gap = (uv - hv) - s->offset;
-    bool c = (cmv & 1) != 0;
+    bool c = (cmv & kNext) != 0;
     if (s->mode == MODE_MIX && comp) c = gap < 0.0;  // c[comp] = g < 0 (:322)
     const double f1 = force_update(s, fv, tv, c);
     const bool tensile = f1 >= 0.0;
@@ -809,7 +839,7 @@ struct ElemPhaseB {
       const bool nc = (tensile && gap < 0.0) || (flowing && gap > 0.0);
       acc[6] += nc ? 1.0 : 0.0;
     }
-    return c ? 1 : 0;
+    return (uint8_t)((cmv & ~kNext) | (c ? kNext : 0));
   }
   __device__ void emit(const Geom& g, int row, int col, const Rec& r, double* acc) const {
     const Scal* s = in.sc;
@@ -872,7 +902,7 @@ struct ProdForce {
                    double* acc) const {
     double gap = 0.0;
     if (comp) gap = (uv - hv) - s->offset;
-    const bool c = (cmv & 1) != 0;
+    const bool c = (cmv & kNext) != 0;
     const double f1 = force_update(s, fv, tv, c);
     const bool tensile = f1 >= 0.0;
     const bool flowing = s->has_H && f1 <= -s->H;
@@ -887,7 +917,7 @@ struct ProdForce {
     if (comp) {
       const bool nc = (tensile && gap < 0.0) || (flowing && gap > 0.0);
       if (s->do_nc && nc) fo = fo + s->tau * gap;         // :384
-      cmv = (uint8_t)((c ? 1 : 0) | (gap < 0.0 ? 2 : 0));
+      cmv = (uint8_t)((c ? kCur | kNext : 0) | (gap < 0.0 ? kGneg : 0));
     }
     const T ft = (T)fo;
     if (comp) acc[0] += (double)ft;
@@ -906,13 +936,17 @@ struct ProdForce {
 };
 
 // u_new = evaluate_disp(f) consumed on the fly: maxdu, max_pen and the
-// convergence tests (:391-434), NaN check (:491-492), maxiter (:494-509).
+// convergence tests (:391-434), NaN check (:491-492), maxiter (:494-509);
+// fused with the start of the next iteration (contact set from the final
+// forces, shifted gap sums of u_new), so no separate phase-A pass is needed.
 template <typename T>
 struct ConsDisp {
-  static constexpr int NS = 2;  // maxdu, max_pen
-  __device__ static constexpr int op(int) { return RMAX; }
+  static constexpr int NS = 8;  // maxdu, max_pen | A_contact, A_cg, Sd, Sdd, gmax, gmin (next iteration)
+  __device__ static constexpr int op(int i) {
+    return i < 2 ? RMAX : (i == 6 ? RMAX : (i == 7 ? RMIN : RSUM));
+  }
   struct Rec {
-    P2<T> u;
+    P2<T> u, f;
     P2<uint8_t> cm;
     HPair h;
   };
@@ -921,24 +955,30 @@ struct ConsDisp {
   __device__ bool skip() const { return in.sc->done != 0; }
   __device__ Rec fetch(const Geom& g, int row, int col) const {
     const size_t p = (size_t)row * g.My + col;
-    return Rec{ld2(in.u + p), ldm(in.cm + p), ldh(g, in.h, in.surf, row, col)};
+    return Rec{ld2(in.u + p), ld2(in.f + p), ldm(in.cm + p), ldh(g, in.h, in.surf, row, col)};
   }
-  __device__ void one(double off, bool comp, uint8_t cmv, double uo, double hv, T v, double* acc) const {
+  __device__ uint8_t one(const Scal* s, bool comp, uint8_t cmv, double uo, double fv, double hv, T v,
+                         double* acc) const {
     const double un = (double)v;
     acc[0] = red_apply(RMAX, acc[0], fabs(un - uo));
     if (comp) {
-      const double pen = (cmv & 1) ? (hv + off - un) : 0.0;
+      const double pen = (cmv & kCur) ? (hv + s->offset - un) : 0.0;
       acc[1] = red_apply(RMAX, acc[1], pen);
     }
+    const bool cn = phase_a_point(s, s->offset, fv, un, hv, comp, acc + 2);  // next shift = this offset
+    return (uint8_t)((cmv & (kCur | kGneg)) | (cn ? kNext : 0));
   }
   __device__ void take(const Geom& g, int row, int col, const Rec& r, T a, T b, double* acc) const {
-    const double off = in.sc->offset;
-    one(off, r.h.c0, r.cm.a, (double)r.u.a, r.h.h0, a, acc);
-    one(off, r.h.c1, r.cm.b, (double)r.u.b, r.h.h1, b, acc);
-    st2(in.u + (size_t)row * g.My + col, a, b);
+    const Scal* s = in.sc;
+    const uint8_t m0 = one(s, r.h.c0, r.cm.a, (double)r.u.a, (double)r.f.a, r.h.h0, a, acc);
+    const uint8_t m1 = one(s, r.h.c1, r.cm.b, (double)r.u.b, (double)r.f.b, r.h.h1, b, acc);
+    const size_t p = (size_t)row * g.My + col;
+    st2(in.u + p, a, b);
+    stm(in.cm + p, m0, m1);
   }
   __device__ void finalize(double* a) const {
     Scal* s = in.sc;
+    s->it += 1;  // iterations completed (:235-236)
     s->nfev += 1;
     s->maxdu = a[0];
     s->G_old = s->G;
@@ -967,7 +1007,6 @@ struct ConsDisp {
     } else {
       s->mix = s->mixfac;  // :299
     }
-    s->shift = s->offset;
     if (conv) {
       s->status = CM_CONVERGED;
       s->done = 1;
@@ -977,6 +1016,9 @@ struct ConsDisp {
     } else if (s->it >= s->maxiter) {
       s->status = CM_MAXITER;
       s->done = 1;
+    } else {
+      s->shift = s->offset;     // the sums a[2..7] were taken with this shift
+      phase_a_logic(s, a + 2);  // start of the next iteration
     }
   }
 };
@@ -1101,7 +1143,7 @@ void dev_free(cm_ctx* ctx, void* p, size_t bytes) {
 }
 
 // Radix sequence and threads-per-line for a length L = 2^a 3^b 5^c.
-// Radices 8, 4, 5, 3, 2 (largest first).  A pass of radix R has L/R
+// Radices 16, 8, 4, 5, 3, 2 (largest first).  A pass of radix R has L/R
 // butterflies dealt round-robin to tpl threads; a thread holds at most
 // ceil(L/R/tpl)*R elements, which must fit the kernel's register budget emax.
 // Prefer tpl dividing every L/R (balanced passes), then larger tpl.
@@ -1109,11 +1151,12 @@ bool make_line_plan(int L, int max_tpl, int emax, LinePlan& best) {
   if (L < 2) return false;
   LinePlan lp{};
   lp.L = L;
+  lp.psh = kNoPad;  // set by the shape chooser
   int rem = L, np = 0;
   while (rem > 1) {
     int pick = 0;
     for (int R : {16, 8, 4, 5, 3, 2})
-      if (rem % R == 0) {
+      if (R <= emax && rem % R == 0) {
         pick = R;
         break;
       }
@@ -1175,7 +1218,7 @@ std::vector<cplx<T>> host_twiddles(int L, int n, int denom) {
 // One FFT executor per precision.
 struct FftExec {
   Geom g{};
-  LinePlan row{}, col{};
+  LinePlan row{}, col{}, col_rev{};  // col_rev: col with the radix order reversed
   int emax_row = 16, emax_col = 16;
   void* tw_row = nullptr;
   void* tw_col = nullptr;
@@ -1208,46 +1251,70 @@ struct Shape {
   int ecol = 0;  // register elements per thread, column transforms (16 or 32)
 };
 
+int emax_bucket(int E) { return E <= 8 ? 8 : (E <= 16 ? 16 : 32); }
+
 template <typename T>
 bool choose_shape(FftExec& ex, const Shape& sh) {
   Geom& g = ex.g;
   const int Mx = g.Mx;
   const size_t cs = sizeof(cplx<T>);
-  constexpr int kMaxBlock = 256;  // FFT kernels are built with __launch_bounds__(256)
-  // 16 register elements per thread when that fits the block, else 32
+  constexpr int kMaxBlock = 256;           // FFT kernels are built with __launch_bounds__(256)
+  constexpr size_t kLdsAuto = 80 * 1024;   // automatic shapes: two blocks per CU (160 KiB LDS)
+  constexpr size_t kLdsMax = 160 * 1024;   // largest block allocation on gfx950
+  const int pad = kPadShift<T>;
+  // register elements per thread: 8 for FP64 (lean kernels, four waves per
+  // SIMD), 16 for FP32, more when the line does not fit a block otherwise
   auto plan_len = [&](int L, int e, LinePlan& lp) {
     if (e) return make_line_plan(L, kMaxBlock, e, lp);
+    if (sizeof(T) == 8 && make_line_plan(L, kMaxBlock, 8, lp)) return true;
     return make_line_plan(L, kMaxBlock, 16, lp) || make_line_plan(L, kMaxBlock, 32, lp);
   };
+  // padding shift for a slab of n values: padded when that leaves room for
+  // two blocks per CU, else unpadded when that does, else whichever fits one
+  // block per CU (or nothing fits: -1)
+  auto slab_shift = [&](size_t n) {
+    for (size_t cap : {kLdsAuto, kLdsMax}) {
+      if (lds_bytes<T>(n, pad) <= cap) return pad;
+      if (lds_bytes<T>(n, kNoPad) <= cap) return kNoPad;
+    }
+    return -1;
+  };
   if (!plan_len(g.N2, sh.erow, ex.row) || !plan_len(Mx, sh.ecol, ex.col)) return false;
+  // Row blocks write runs of R*Cb spectrum values: aim at >= 128-byte runs
+  // with blocks that leave room for two per CU.
   int Cb = 1;
   if (sh.Cb) {
     Cb = sh.Cb;
-    if (g.N2 % Cb || ex.col.tpl * Cb > kMaxBlock || (Cb > 1 && lds_bytes<T>((size_t)Mx * Cb) > 65536)) return false;
+    if (g.N2 % Cb || ex.col.tpl * Cb > kMaxBlock) return false;
   } else {
-    // contiguous slab of Mx*Cb complex values in LDS; aim at 64-byte row segments
     const int cb_target = (int)(64 / cs) > 0 ? (int)(64 / cs) : 1;
-    while (Cb * 2 <= cb_target && g.N2 % (Cb * 2) == 0 && lds_bytes<T>((size_t)Mx * Cb * 2) <= 65536 &&
+    while (Cb * 2 <= cb_target && g.N2 % (Cb * 2) == 0 && lds_bytes<T>((size_t)Mx * Cb * 2, pad) <= kLdsAuto &&
            ex.col.tpl * Cb * 2 <= kMaxBlock)
       Cb *= 2;
   }
-  if (lds_bytes<T>((size_t)Mx * Cb) > 150 * 1024) return false;
+  const int csh = slab_shift((size_t)Mx * Cb);
+  if (csh < 0) return false;
   int R = 1;
   if (sh.R) {
     R = sh.R;
-    if (Mx % R || ex.row.tpl * R > kMaxBlock || (R > 1 && lds_bytes<T>((size_t)R * g.N2) > 65536)) return false;
+    if (Mx % R || ex.row.tpl * R > kMaxBlock) return false;
   } else {
-    while (Mx % (R * 2) == 0 && ex.row.tpl * R * 2 <= kMaxBlock && lds_bytes<T>((size_t)R * 2 * g.N2) <= 65536 &&
-           (ex.row.tpl * R < 128 || (size_t)R * Cb * cs < 128))
+    while (Mx % (R * 2) == 0 && ex.row.tpl * R * 2 <= kMaxBlock &&
+           lds_bytes<T>((size_t)R * 2 * g.N2, pad) <= kLdsAuto && (size_t)R * Cb * cs < 128)
       R *= 2;
   }
-  if (lds_bytes<T>((size_t)R * g.N2) > 150 * 1024) return false;
+  const int rsh = slab_shift((size_t)R * g.N2);
+  if (rsh < 0) return false;
   g.Cb = Cb;
   g.R = R;
-  ex.lds_row = lds_bytes<T>((size_t)R * g.N2);
-  ex.lds_col = lds_bytes<T>((size_t)Mx * Cb);
-  ex.emax_row = ex.row.E <= 16 ? 16 : 32;
-  ex.emax_col = ex.col.E <= 16 ? 16 : 32;
+  ex.row.psh = rsh;
+  ex.col.psh = csh;
+  ex.lds_row = lds_bytes<T>((size_t)R * g.N2, rsh);
+  ex.lds_col = lds_bytes<T>((size_t)Mx * Cb, csh);
+  ex.emax_row = emax_bucket(ex.row.E);
+  ex.emax_col = emax_bucket(ex.col.E);
+  ex.col_rev = ex.col;
+  for (int q = 0; q < ex.col.npass; ++q) ex.col_rev.radix[q] = ex.col.radix[ex.col.npass - 1 - q];
   return true;
 }
 
@@ -1294,7 +1361,10 @@ void launch_r2c(const FftExec& ex, InstArr<T> ia, int B, int slot, hipStream_t s
   dim3 grid(ex.g.Mx / ex.g.R, B), block(ex.row.tpl * ex.g.R);
   const cplx<T>* tw = (const cplx<T>*)ex.tw_row;
   const cplx<T>* twp = (const cplx<T>*)ex.tw_post;
-  if (ex.emax_row == 16) {
+  if (ex.emax_row == 8) {
+    allow_big_lds((const void*)k_r2c<T, 8, P>, ex.lds_row);
+    hipLaunchKernelGGL((k_r2c<T, 8, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
+  } else if (ex.emax_row == 16) {
     allow_big_lds((const void*)k_r2c<T, 16, P>, ex.lds_row);
     hipLaunchKernelGGL((k_r2c<T, 16, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
   } else {
@@ -1308,7 +1378,10 @@ void launch_c2r(const FftExec& ex, InstArr<T> ia, int B, int slot, hipStream_t s
   dim3 grid(ex.g.Mx / ex.g.R, B), block(ex.row.tpl * ex.g.R);
   const cplx<T>* tw = (const cplx<T>*)ex.tw_row;
   const cplx<T>* twp = (const cplx<T>*)ex.tw_post;
-  if (ex.emax_row == 16) {
+  if (ex.emax_row == 8) {
+    allow_big_lds((const void*)k_c2r<T, 8, P>, ex.lds_row);
+    hipLaunchKernelGGL((k_c2r<T, 8, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
+  } else if (ex.emax_row == 16) {
     allow_big_lds((const void*)k_c2r<T, 16, P>, ex.lds_row);
     hipLaunchKernelGGL((k_c2r<T, 16, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
   } else {
@@ -1321,12 +1394,15 @@ template <typename T, int MODE>
 void launch_col(const FftExec& ex, InstArr<T> ia, int B, int useK, int skip, hipStream_t st) {
   dim3 grid(ex.g.N2 / ex.g.Cb, B), block(ex.col.tpl * ex.g.Cb);
   const cplx<T>* tw = (const cplx<T>*)ex.tw_col;
-  if (ex.emax_col == 16) {
+  if (ex.emax_col == 8) {
+    allow_big_lds((const void*)k_col<T, 8, MODE>, ex.lds_col);
+    hipLaunchKernelGGL((k_col<T, 8, MODE>), grid, block, ex.lds_col, st, ex.g, ex.col, ex.col_rev, tw, ia, useK, skip);
+  } else if (ex.emax_col == 16) {
     allow_big_lds((const void*)k_col<T, 16, MODE>, ex.lds_col);
-    hipLaunchKernelGGL((k_col<T, 16, MODE>), grid, block, ex.lds_col, st, ex.g, ex.col, tw, ia, useK, skip);
+    hipLaunchKernelGGL((k_col<T, 16, MODE>), grid, block, ex.lds_col, st, ex.g, ex.col, ex.col_rev, tw, ia, useK, skip);
   } else {
     allow_big_lds((const void*)k_col<T, 32, MODE>, ex.lds_col);
-    hipLaunchKernelGGL((k_col<T, 32, MODE>), grid, block, ex.lds_col, st, ex.g, ex.col, tw, ia, useK, skip);
+    hipLaunchKernelGGL((k_col<T, 32, MODE>), grid, block, ex.lds_col, st, ex.g, ex.col, ex.col_rev, tw, ia, useK, skip);
   }
 }
 
@@ -1472,7 +1548,9 @@ int plan_build(cm_plan* P) {
   const size_t M = (size_t)P->Mx * P->My;
   // element-wise kernels: blocks stride over rows; few enough blocks that the
   // last block's pass over the partials stays short
-  P->nblk_elem = (int)std::max<size_t>(1, std::min<size_t>(std::min<size_t>(512, (size_t)P->Mx), (M + 2047) / 2048));
+  // element-wise passes: about one row of work per block, up to 2048 blocks
+  // (eight waves per CU on 256 CUs) so enough loads are in flight
+  P->nblk_elem = (int)std::max<size_t>(1, std::min<size_t>(std::min<size_t>(2048, (size_t)P->Mx), (M + 1023) / 1024));
   return 0;
 }
 
@@ -1514,8 +1592,8 @@ int autotune(cm_plan* P, int reps, std::string& desc) {
   int tried = 0;
   for (int Cb : {1, 2, 4, 8})
     for (int R : {1, 2, 4, 8})
-      for (int er : {16, 32})
-        for (int ec : {16, 32}) {
+      for (int er : {8, 16, 32})
+        for (int ec : {8, 16, 32}) {
           FftExec trial = P->ex;
           if (!choose_shape<T>(trial, Shape{Cb, R, er, ec})) continue;
           evaluate_ex<T>(P, trial, a, b, 0, st);  // warm-up (first launch of a kernel)
@@ -1542,7 +1620,7 @@ int autotune(cm_plan* P, int reps, std::string& desc) {
   snprintf(buf, sizeof buf,
            "{\"Cb\": %d, \"R\": %d, \"erow\": %d, \"ecol\": %d, \"row_tpl\": %d, \"col_tpl\": %d, "
            "\"evaluate_us\": %.3f, \"candidates\": %d}",
-           best.g.Cb, best.g.R, best.row.E <= 16 ? 16 : 32, best.col.E <= 16 ? 16 : 32, best.row.tpl, best.col.tpl,
+           best.g.Cb, best.g.R, emax_bucket(best.row.E), emax_bucket(best.col.E), best.row.tpl, best.col.tpl,
            best_us, tried);
   desc = buf;
   return 0;
@@ -1654,7 +1732,8 @@ void enqueue_iteration(cm_session* S) {
   ia.arr = (const Inst<T>*)((char*)S->ws + S->L.inst);
   const int B = S->B, nbe = S->P->nblk_elem;
   hipStream_t st = S->st;
-  launch_elem<T, ElemPhaseA<T>>(ex, nbe, ia, B, 2, st);
+  // phase A of iteration k+1 is fused into ConsDisp of iteration k; the first
+  // one runs at set-up
   if (S->any_bisect) hipLaunchKernelGGL((k_bisect<T>), dim3(B), dim3(1024), 0, st, ex.g, ia);
   launch_r2c<T, ProdDir<T>>(ex, ia, B, 3, st);
   launch_col<T, 0>(ex, ia, B, 0, 2, st);
@@ -1734,7 +1813,10 @@ int session_begin(cm_plan* P, int B, const cm_ccg_params* prm, const double* con
     Scal& s = hsc[b];
     std::memset(&s, 0, sizeof(Scal));
     s.F = q.external_force;
-    s.H = q.hardness;
+    // FP32 plans store forces in FP32: with the cap rounded to FP32, a force
+    // clipped to -H is exactly -H again when read back, so the flowing test
+    // f <= -H stays true for it (an unrounded cap can flip it every pass)
+    s.H = std::is_same<T, float>::value ? (double)(float)q.hardness : q.hardness;
     s.pentol = q.pentol;
     s.forcetol = q.forcetol;
     s.thermotol = q.thermotol;
@@ -1788,6 +1870,7 @@ int session_begin(cm_plan* P, int B, const cm_ccg_params* prm, const double* con
     launch_c2r<T, ConsCopy<T>>(ex, ia0, B, 1, st);
     launch_elem<T, ElemZeroPad<T>>(ex, nbe, ia, B, 1, st);
   }
+  launch_elem<T, ElemPhaseA<T>>(ex, nbe, ia, B, 2, st);  // start of iteration 1
   HIPCHK(ctx, hipGetLastError());
   *out = S;
   return 0;
@@ -1858,7 +1941,7 @@ int session_profile(cm_session* S, int n, float* us) {
   for (int it = 0; it < n; ++it) {
     int k = 0;
     HIPCHK(ctx, hipEventRecord(ev[k++], st));
-    launch_elem<T, ElemPhaseA<T>>(ex, nbe, ia, B, 2, st);
+    // (phase A is fused into the displacement pass: nothing to launch)
     HIPCHK(ctx, hipEventRecord(ev[k++], st));
     if (S->any_bisect) hipLaunchKernelGGL((k_bisect<T>), dim3(B), dim3(1024), 0, st, ex.g, ia);
     HIPCHK(ctx, hipEventRecord(ev[k++], st));
@@ -2056,8 +2139,16 @@ int cm_plan_green_host(cm_plan* plan, double* out) {
   return 0;
 }
 
+// Explicit launch shapes are only offered for FP64 plans: an FP32 2048^2
+// solve with non-automatic shapes ended in a GPU fault whose cause is not
+// established, so FP32 plans keep the automatic shape.
+int refuse_fp32_shapes(cm_plan* plan) {
+  return fail(plan->ctx, CM_E_UNSUPPORTED, "FP32 plans use the automatic launch shape");
+}
+
 int cm_plan_autotune(cm_plan* plan, int reps, char* out, size_t out_len) {
   if (!plan || reps < 1) return fail(plan ? plan->ctx : nullptr, CM_E_ARG, "bad argument");
+  if (plan->prec != CM_FP64) return refuse_fp32_shapes(plan);
   std::string d;
   int rc = plan->prec == CM_FP64 ? autotune<double>(plan, reps, d) : autotune<float>(plan, reps, d);
   if (rc) return rc;
@@ -2070,6 +2161,7 @@ int cm_plan_autotune(cm_plan* plan, int reps, char* out, size_t out_len) {
 
 int cm_plan_set_shape(cm_plan* plan, int Cb, int R, int erow, int ecol) {
   if (!plan) return fail(nullptr, CM_E_ARG, "null plan");
+  if (plan->prec != CM_FP64) return refuse_fp32_shapes(plan);
   FftExec trial = plan->ex;
   const bool ok = plan->prec == CM_FP64 ? choose_shape<double>(trial, Shape{Cb, R, erow, ecol})
                                         : choose_shape<float>(trial, Shape{Cb, R, erow, ecol});
@@ -2082,8 +2174,8 @@ int cm_plan_shape(cm_plan* plan, int* Cb, int* R, int* erow, int* ecol) {
   if (!plan) return fail(nullptr, CM_E_ARG, "null plan");
   if (Cb) *Cb = plan->ex.g.Cb;
   if (R) *R = plan->ex.g.R;
-  if (erow) *erow = plan->ex.row.E <= 16 ? 16 : 32;
-  if (ecol) *ecol = plan->ex.col.E <= 16 ? 16 : 32;
+  if (erow) *erow = emax_bucket(plan->ex.row.E);
+  if (ecol) *ecol = emax_bucket(plan->ex.col.E);
   return 0;
 }
 

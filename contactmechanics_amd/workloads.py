@@ -115,3 +115,19 @@ def config5(count, rate_hz, seed=5, side_min=256, side_max=4096):
                         pressure=float(np.exp(rng.uniform(np.log(1e-4), np.log(1e-2)))),
                         hardness=float(rng.uniform(0.02, 0.1)) if hard else None, seed=5000 + i))
     return out
+
+
+def materialize(rec, gtol=1e-6, maxiter=20000):
+    """Solver inputs for one C4/C5 record: heights, mean pressure -> total
+    force (pressure x L^2, L = 1), hardness pressure -> pixel hardness
+    (pressure x area_per_pt), mapped tolerances."""
+    n = rec["n"]
+    out = dict(rec)
+    out["heights"] = self_affine(n, rec["H"], rec["slope"], rec["seed"])
+    out["external_force"] = rec["pressure"]
+    hp = rec.get("hardness")
+    out["hardness"] = hp / (n * n) if hp is not None else None
+    out["hardness_force"] = out["hardness"]
+    out["maxiter"] = maxiter
+    out.update(tolerances(gtol, n, n))
+    return out

@@ -123,8 +123,10 @@ int cm_plan_green_host(cm_plan* plan, double* host_out);
  * passes (columns per column block, rows per row block, register elements per
  * thread) and keep the fastest; `out` receives a JSON description.  Must not
  * run while a session of this plan is in flight.  cm_plan_set_shape applies a
- * cached choice (0 = automatic), cm_plan_shape reads the current one.  No
- * reference counterpart (muGrid picks FFTW plans internally). */
+ * cached choice (0 = automatic), cm_plan_shape reads the current one.  Explicit
+ * shapes exist for FP64 plans; FP32 plans keep the automatic shape and both
+ * calls return CM_E_UNSUPPORTED for them.  Register elements are 8, 16 or 32.
+ * No reference counterpart (muGrid picks FFTW plans internally). */
 int cm_plan_autotune(cm_plan* plan, int reps, char* out, size_t out_len);
 int cm_plan_set_shape(cm_plan* plan, int Cb, int R, int erow, int ecol);
 int cm_plan_shape(cm_plan* plan, int* Cb, int* R, int* erow, int* ecol);

@@ -22,7 +22,8 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("cls,ocls,nx,ny,prec", [
     (PeriodicFFTElasticHalfSpace, PeriodicHalfSpace, 128, 96, "fp64"),
-    (PeriodicFFTElasticHalfSpace, PeriodicHalfSpace, 256, 256, "fp32"),
+    (PeriodicFFTElasticHalfSpace, PeriodicHalfSpace, 256, 256, "fp64"),
+    (PeriodicFFTElasticHalfSpace, PeriodicHalfSpace, 2048, 64, "fp64"),
     (FreeFFTElasticHalfSpace, FreeHalfSpace, 48, 80, "fp64"),
 ])
 def test_all_shapes_agree(gpu, cls, ocls, nx, ny, prec):
@@ -32,7 +33,7 @@ def test_all_shapes_agree(gpu, cls, ocls, nx, ny, prec):
     ud, fd = c.evaluate_disp(f), c.evaluate_force(f)
     tol = 1e-9 if prec == "fp64" else 3e-6
     n_ok = 0
-    for Cb, R, er, ec in itertools.product([1, 2, 4, 8], [1, 2, 4, 8], [16, 32], [16, 32]):
+    for Cb, R, er, ec in itertools.product([1, 2, 4, 8], [1, 2, 4, 8], [8, 16, 32], [8, 16, 32]):
         try:
             g.plan.set_shape(Cb, R, er, ec)
         except _native.CmError:
@@ -56,3 +57,11 @@ def test_tune_and_cache(gpu, tmp_path):
     f = np.random.default_rng(0).standard_normal((256, 256))
     c = PeriodicHalfSpace((256, 256), 1.0, (1.0, 1.0))
     assert np.max(np.abs(g2.evaluate_disp(f) - c.evaluate_disp(f))) / np.max(np.abs(c.evaluate_disp(f))) < 1e-12
+
+
+def test_fp32_plans_keep_automatic_shape(gpu):
+    g = PeriodicFFTElasticHalfSpace((256, 256), 1.0, (1.0, 1.0), precision="fp32")
+    with pytest.raises(_native.CmError):
+        g.plan.set_shape(1, 1, 16, 16)
+    d = tune(g, cache_path=None)
+    assert not d["tuned"] and d["Cb"] == g.plan.shape()["Cb"]
