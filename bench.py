@@ -53,6 +53,25 @@ def canonical_iteration_bytes(M, Mt, w, free):
     return w * (12 * M + 6 * M + Mt + (M // 4 if free else 0))
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_ccg_c2_2048_periodic.json")
+
+
+def measured_traffic(role, precision="fp64"):
+    """HBM bytes per launch of a kernel role from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (already scaled by the copy-kernel factors);
+    None when no such profile is present."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            d = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    w = "double" if precision == "fp64" else "float"
+    for name, k in d.get("kernels", {}).items():
+        if k.get("role") == role and f"<{w}" in name:
+            return k["fetch_bytes"] + k["write_bytes"]
+    return None
+
+
 def throughput(sub, heights, F, steps, warmup, precision):
     import torch
     from contactmechanics_amd.session import CcgSession, make_params
@@ -326,7 +345,7 @@ def main():
         "config": {"workload": cfg["name"], "grid": [n, n], "seed": 2048, "external_force": 0.01,
                    "timed_region": "resident session, exact --steps iterations, tolerances unreachable"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8000.0, "traffic": None,
+                     "frac": achieved / 8000.0, "traffic": measured_traffic(dom.rstrip("12")),
                      "bytes_per_launch": b, "avg_us": ku[dom]},
         "cpu_baseline": cpu,
         "kernel_us_fp64": {k: round(v, 2) for k, v in ku.items()},

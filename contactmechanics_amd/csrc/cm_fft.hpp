@@ -173,6 +173,7 @@ struct LinePlan {
   int E;          // most elements a thread holds in any pass
   int npass;      // number of radix passes
   int psh;        // LDS padding shift (kPadShift or kNoPad)
+  int rlast;      // radix[npass - 1] (read without indexing the array at run time)
   int radix[14];  // radix of each pass (product == L)
 };
 
@@ -351,7 +352,7 @@ __device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const
         if (e / R < nb) v[e] = lds[lpad(base + slot_element(e, R, p.L, tl, p.tpl) * stride, p.psh)];
       return;
     }
-    switch (p.radix[p.npass - 1]) {
+    switch (p.rlast) {
       case 2: last_pass_regs<T, 2, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw, p.psh); break;
       case 3: last_pass_regs<T, 3, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw, p.psh); break;
       case 4: last_pass_regs<T, 4, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw, p.psh); break;

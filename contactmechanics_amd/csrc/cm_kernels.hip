@@ -80,7 +80,7 @@ struct Inst {
   T* dst;          // operator output (plain evaluate)
   Scal* sc;
   double* part;    // per-block partial reductions
-  unsigned* ctr;   // last-block tickets (8 slots)
+  unsigned* ctr;   // last-block tickets (8 slots of kCtrSlot counters)
 };
 
 template <typename T>
@@ -103,6 +103,14 @@ __device__ __forceinline__ void wait_stores_done() { asm volatile("s_waitcnt vmc
 // see data written through by other XCDs.
 __device__ __forceinline__ void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
+// Tickets are taken in kTicketGroups groups (block b in group b % groups),
+// each counter on its own 64-byte line: the block completing a group takes a
+// ticket on the top counter.  Same-address atomics serialise (about 6 ns each
+// on gfx950), so one counter for 1-2k blocks would add 6-12 us per kernel.
+constexpr int kTicketGroups = 16;
+constexpr int kCtrLine = 16;                             // unsigned per 64-byte line
+constexpr int kCtrSlot = kCtrLine * (kTicketGroups + 1); // counters of one reduction slot
+
 // Cross-block reduction: every block reduces NS slots, writes its partials,
 // takes a ticket; the block drawing the last ticket reduces all partials in a
 // fixed order (deterministic result) and returns true.
@@ -121,13 +129,23 @@ __device__ bool reduce_blocks(double* acc, double* part, unsigned* ctr, int nblo
 #pragma unroll
   for (int i = 0; i < NS; ++i) ops[i] = P::op(i);
   block_reduce<NS>(acc, ops, scr);
+  // partials are slot-major, part[i * nblocks + b], so the last block reads
+  // each slot with coalesced loads
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int i = 0; i < NS; ++i)
-      __hip_atomic_store(&part[(size_t)bid * NS + i], acc[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&part[(size_t)i * nblocks + bid], acc[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     wait_stores_done();
-    unsigned tk = atomicAdd(ctr, 1u);
-    s_last = (tk == (unsigned)(nblocks - 1));
+    const int g = bid % kTicketGroups;
+    const int gsize = nblocks / kTicketGroups + (g < nblocks % kTicketGroups ? 1 : 0);
+    const int ngroups = nblocks < kTicketGroups ? nblocks : kTicketGroups;
+    unsigned* grp = ctr + kCtrLine * (1 + g);
+    int last = 0;
+    if (atomicAdd(grp, 1u) == (unsigned)(gsize - 1)) {  // this block completes its group
+      __hip_atomic_store(grp, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      last = atomicAdd(ctr, 1u) == (unsigned)(ngroups - 1);
+    }
+    s_last = last;
   }
   __syncthreads();
   if (!s_last) return false;
@@ -135,11 +153,22 @@ __device__ bool reduce_blocks(double* acc, double* part, unsigned* ctr, int nblo
   double v[NS > 0 ? NS : 1];
 #pragma unroll
   for (int i = 0; i < NS; ++i) v[i] = red_identity(ops[i]);
-  // plain loads: the acquire fence above makes every block's partials
-  // visible, and independent loads can all be in flight at once
-  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+  // plain loads (the acquire above makes every block's partials visible),
+  // kB blocks per thread and slot issued together so they are all in flight
+  constexpr int kB = 4;
+  for (int b0 = threadIdx.x; b0 < nblocks; b0 += kB * blockDim.x) {
+    double w[NS > 0 ? NS : 1][kB];
 #pragma unroll
-    for (int i = 0; i < NS; ++i) v[i] = red_apply(ops[i], v[i], part[(size_t)b * NS + i]);
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+      for (int k = 0; k < kB; ++k) {
+        const int b = b0 + k * (int)blockDim.x;
+        w[i][k] = b < nblocks ? part[(size_t)i * nblocks + b] : red_identity(ops[i]);
+      }
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+      for (int k = 0; k < kB; ++k) v[i] = red_apply(ops[i], v[i], w[i][k]);
   }
   block_reduce<NS>(v, ops, scr);
   if (threadIdx.x == 0) {
@@ -203,14 +232,25 @@ __device__ __forceinline__ HPair ldh(const Geom& g, const T* h, const uint8_t* s
     r.c0 = r.c0 && surf[q];
     r.c1 = r.c1 && surf[q + 1];
   }
-  r.h0 = r.c0 ? (double)h[q] : 0.0;
-  r.h1 = r.c1 ? (double)h[q + 1] : 0.0;
+  if (r.c0 && r.c1 && (g.ny & 1) == 0) {  // both points valid and the pair aligned: one vector load
+    const P2<T> v = ld2(h + q);
+    r.h0 = (double)v.a;
+    r.h1 = (double)v.b;
+  } else {
+    r.h0 = r.c0 ? (double)h[q] : 0.0;
+    r.h1 = r.c1 ? (double)h[q + 1] : 0.0;
+  }
   return r;
 }
 
+// Largest block of an FFT kernel: 512 threads for the lean 8-element builds
+// (about 120 VGPRs: two such blocks fill a CU at four waves per SIMD), 256
+// for the 16/32-element builds.
+__host__ __device__ constexpr int fft_block_max(int emax) { return emax == 8 ? 512 : 256; }
+
 // Row pass: real rows -> packed half spectrum (forward, unnormalised).
 template <typename T, int EMAX, class P>
-__global__ void __launch_bounds__(256) k_r2c(Geom g, LinePlan lp, const cplx<T>* __restrict__ tw,
+__global__ void __launch_bounds__(fft_block_max(EMAX)) k_r2c(Geom g, LinePlan lp, const cplx<T>* __restrict__ tw,
                                              const cplx<T>* __restrict__ twp, InstArr<T> ia, int ctr_slot) {
   extern __shared__ __align__(16) unsigned char smem_r2c[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_r2c);
@@ -276,7 +316,7 @@ __global__ void __launch_bounds__(256) k_r2c(Geom g, LinePlan lp, const cplx<T>*
     S[((size_t)cb * g.Mx + blockIdx.x * g.R + rr) * g.Cb + kk] = lds[lpad(rr * N2 + cb * g.Cb + kk, lp.psh)];
   }
   if constexpr (P::NS > 0) {
-    if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
+    if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
       p.finalize(acc);
   }
 }
@@ -284,7 +324,7 @@ __global__ void __launch_bounds__(256) k_r2c(Geom g, LinePlan lp, const cplx<T>*
 // Column pass.  MODE 0: forward * G * inverse; 1: forward only; 2: inverse only.
 // skip: 0 never, 1 when the solve is done, 2 when done or in a mixing step.
 template <typename T, int EMAX, int MODE>
-__global__ void __launch_bounds__(256) k_col(Geom g, LinePlan lp, LinePlan rv, const cplx<T>* __restrict__ tw,
+__global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp, LinePlan rv, const cplx<T>* __restrict__ tw,
                                               InstArr<T> ia, int useK, int skip) {
   extern __shared__ __align__(16) unsigned char smem_col[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_col);
@@ -304,7 +344,7 @@ __global__ void __launch_bounds__(256) k_col(Geom g, LinePlan lp, LinePlan rv, c
     // forward pass -> x G in registers -> first inverse pass (same radix, same
     // slots), last inverse pass -> HBM.  Two LDS round trips fewer each way.
     const int c = threadIdx.x % Cb, tl = threadIdx.x / Cb;
-    const int R0 = lp.radix[0], Rl = lp.radix[lp.npass - 1];
+    const int R0 = lp.radix[0], Rl = lp.rlast;
     const int nb0 = nbutterflies(Mx, R0, tl, lp.tpl), nbl = nbutterflies(Mx, Rl, tl, lp.tpl);
     cplx<T> v[EMAX];
 #pragma unroll
@@ -360,7 +400,7 @@ __global__ void __launch_bounds__(256) k_col(Geom g, LinePlan lp, LinePlan rv, c
 // Row pass: packed half spectrum -> real rows (inverse, unnormalised; the
 // round trip r2c -> c2r scales by Mx*My, which the kernels G/K absorb).
 template <typename T, int EMAX, class P>
-__global__ void __launch_bounds__(256) k_c2r(Geom g, LinePlan lp, const cplx<T>* __restrict__ tw,
+__global__ void __launch_bounds__(fft_block_max(EMAX)) k_c2r(Geom g, LinePlan lp, const cplx<T>* __restrict__ tw,
                                              const cplx<T>* __restrict__ twp, InstArr<T> ia, int ctr_slot) {
   extern __shared__ __align__(16) unsigned char smem_c2r[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_c2r);
@@ -400,7 +440,7 @@ __global__ void __launch_bounds__(256) k_c2r(Geom g, LinePlan lp, const cplx<T>*
 #pragma unroll
   for (int i = 0; i < P::NS; ++i) acc[i] = red_identity(P::op(i));
   {
-    const int Rl = lp.radix[lp.npass - 1];
+    const int Rl = lp.rlast;
     const int nbl = nbutterflies(N2, Rl, tl, lp.tpl);
 #pragma unroll
     for (int e0 = 0; e0 < EMAX; e0 += kPairs) {
@@ -418,7 +458,7 @@ __global__ void __launch_bounds__(256) k_c2r(Geom g, LinePlan lp, const cplx<T>*
     }
   }
   if constexpr (P::NS > 0) {
-    if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
+    if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
       p.finalize(acc);
   }
 }
@@ -450,7 +490,7 @@ __global__ void __launch_bounds__(256) k_elem(Geom g, InstArr<T> ia, int ctr_slo
     }
   }
   if constexpr (P::NS > 0) {
-    if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
+    if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
       p.finalize(acc);
   }
 }
@@ -1165,6 +1205,7 @@ bool make_line_plan(int L, int max_tpl, int emax, LinePlan& best) {
     rem /= pick;
   }
   lp.npass = np;
+  lp.rlast = lp.radix[np - 1];
   int need = 1;  // smallest tpl meeting the register budget
   for (int s = 0; s < np; ++s) {
     const int R = lp.radix[s];
@@ -1258,16 +1299,15 @@ bool choose_shape(FftExec& ex, const Shape& sh) {
   Geom& g = ex.g;
   const int Mx = g.Mx;
   const size_t cs = sizeof(cplx<T>);
-  constexpr int kMaxBlock = 256;           // FFT kernels are built with __launch_bounds__(256)
   constexpr size_t kLdsAuto = 80 * 1024;   // automatic shapes: two blocks per CU (160 KiB LDS)
   constexpr size_t kLdsMax = 160 * 1024;   // largest block allocation on gfx950
   const int pad = kPadShift<T>;
-  // register elements per thread: 8 for FP64 (lean kernels, four waves per
-  // SIMD), 16 for FP32, more when the line does not fit a block otherwise
+  // register elements per thread: 8 (lean kernels, four or more waves per
+  // SIMD), more when the line does not fit a block otherwise
   auto plan_len = [&](int L, int e, LinePlan& lp) {
-    if (e) return make_line_plan(L, kMaxBlock, e, lp);
-    if (sizeof(T) == 8 && make_line_plan(L, kMaxBlock, 8, lp)) return true;
-    return make_line_plan(L, kMaxBlock, 16, lp) || make_line_plan(L, kMaxBlock, 32, lp);
+    if (e) return make_line_plan(L, fft_block_max(e), e, lp);
+    return make_line_plan(L, fft_block_max(8), 8, lp) || make_line_plan(L, fft_block_max(16), 16, lp) ||
+           make_line_plan(L, fft_block_max(32), 32, lp);
   };
   // padding shift for a slab of n values: padded when that leaves room for
   // two blocks per CU, else unpadded when that does, else whichever fits one
@@ -1280,16 +1320,17 @@ bool choose_shape(FftExec& ex, const Shape& sh) {
     return -1;
   };
   if (!plan_len(g.N2, sh.erow, ex.row) || !plan_len(Mx, sh.ecol, ex.col)) return false;
+  const int row_max = fft_block_max(emax_bucket(ex.row.E)), col_max = fft_block_max(emax_bucket(ex.col.E));
   // Row blocks write runs of R*Cb spectrum values: aim at >= 128-byte runs
   // with blocks that leave room for two per CU.
   int Cb = 1;
   if (sh.Cb) {
     Cb = sh.Cb;
-    if (g.N2 % Cb || ex.col.tpl * Cb > kMaxBlock) return false;
+    if (g.N2 % Cb || ex.col.tpl * Cb > col_max) return false;
   } else {
     const int cb_target = (int)(64 / cs) > 0 ? (int)(64 / cs) : 1;
     while (Cb * 2 <= cb_target && g.N2 % (Cb * 2) == 0 && lds_bytes<T>((size_t)Mx * Cb * 2, pad) <= kLdsAuto &&
-           ex.col.tpl * Cb * 2 <= kMaxBlock)
+           ex.col.tpl * Cb * 2 <= col_max)
       Cb *= 2;
   }
   const int csh = slab_shift((size_t)Mx * Cb);
@@ -1297,9 +1338,9 @@ bool choose_shape(FftExec& ex, const Shape& sh) {
   int R = 1;
   if (sh.R) {
     R = sh.R;
-    if (Mx % R || ex.row.tpl * R > kMaxBlock) return false;
+    if (Mx % R || ex.row.tpl * R > row_max) return false;
   } else {
-    while (Mx % (R * 2) == 0 && ex.row.tpl * R * 2 <= kMaxBlock &&
+    while (Mx % (R * 2) == 0 && ex.row.tpl * R * 2 <= row_max &&
            lds_bytes<T>((size_t)R * 2 * g.N2, pad) <= kLdsAuto && (size_t)R * Cb * cs < 128)
       R *= 2;
   }
@@ -1315,6 +1356,7 @@ bool choose_shape(FftExec& ex, const Shape& sh) {
   ex.emax_col = emax_bucket(ex.col.E);
   ex.col_rev = ex.col;
   for (int q = 0; q < ex.col.npass; ++q) ex.col_rev.radix[q] = ex.col.radix[ex.col.npass - 1 - q];
+  ex.col_rev.rlast = ex.col_rev.radix[ex.col.npass - 1];
   return true;
 }
 
@@ -1662,7 +1704,7 @@ WsLayout ws_layout(const cm_plan* P, int B) {
   L.st_S = al256((M / 2) * 2 * w);
   L.st_sc = al256(sizeof(Scal));
   L.st_part = al256(nparts * sizeof(double));
-  L.st_ctr = al256(8 * sizeof(unsigned));
+  L.st_ctr = al256(8 * kCtrSlot * sizeof(unsigned));
   L.st_hc = w == 4 ? al256((size_t)P->d.nx * P->d.ny * 4) : 0;  // FP32 copy of the heights
   const size_t inst_bytes = al256(std::max(sizeof(Inst<double>), sizeof(Inst<float>)) * B);
   size_t o = 0;
