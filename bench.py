@@ -38,8 +38,10 @@ def kernel_bytes(name, M, Mt, w):
         "phase_a": 0,
         "bisect": 0,
         "dir_r2c": M * (4 * w + 1) + hw * Mt,
-        "col_conv1": int(2.5 * w * M),
-        "tau_c2r": M * (2 * w + 1),
+        # forward column FFT of the direction and sum_k G|T|^2 (reads S and G,
+        # writes nothing): the step length needs no inverse pass
+        "col_conv1": int(1.5 * w * M),
+        "tau_c2r": 0,
         "phase_b": M * (3 * w + 1) + hw * Mt,
         "force_r2c": M * (5 * w + 2) + hw * Mt,
         "col_conv2": int(2.5 * w * M),

@@ -153,16 +153,12 @@ template <typename T, bool INV> struct Dft<T, 16, INV> {
 };
 
 // LDS keeps complex values at padded positions: one slot is skipped every
-// 128 bytes (shift kPadShift<T>), so the stride-R Stockham writes of the early
-// passes spread over all banks instead of piling onto a few.  A line whose
-// padded slab would not fit 64 KiB runs unpadded (shift kNoPad).
+// 128 bytes, so the stride-R Stockham writes of the early passes spread over
+// all banks instead of piling onto a few.
 template <typename T> constexpr int kPadShift = sizeof(cplx<T>) == 16 ? 3 : 4;
-constexpr int kNoPad = 30;
-// (the explicit unpadded branch also keeps the gfx950 backend away from a
-// register-class error it hits with a plain variable shift here)
-__device__ __forceinline__ int lpad(int i, int psh) { return psh == kNoPad ? i : i + (i >> psh); }
-template <typename T> inline size_t lds_bytes(size_t n, int psh) {
-  return (n + (n >> psh)) * sizeof(cplx<T>);
+template <typename T> __device__ __forceinline__ int lpad(int i) { return i + (i >> kPadShift<T>); }
+template <typename T> inline size_t lds_bytes(size_t n) {
+  return (n + (n >> kPadShift<T>)) * sizeof(cplx<T>);
 }
 
 // ---------------------------------------------------------------------------
@@ -172,7 +168,6 @@ struct LinePlan {
   int tpl;        // threads per line
   int E;          // most elements a thread holds in any pass
   int npass;      // number of radix passes
-  int psh;        // LDS padding shift (kPadShift or kNoPad)
   int rlast;      // radix[npass - 1] (read without indexing the array at run time)
   int radix[14];  // radix of each pass (product == L)
 };
@@ -182,7 +177,7 @@ struct LinePlan {
 // Line element n lives at lds[base + n*stride].
 template <typename T, int R, bool INV, int EMAX>
 __device__ __forceinline__ void stockham_pass(cplx<T>* lds, int base, int stride, int L, int Ns,
-                                              int tl, int tpl, int nb, const cplx<T>* __restrict__ tw, int psh) {
+                                              int tl, int tpl, int nb, const cplx<T>* __restrict__ tw) {
   constexpr int NBMAX = EMAX / R > 0 ? EMAX / R : 1;
   cplx<T> v[NBMAX][R];
   const int LR = L / R;
@@ -191,46 +186,65 @@ __device__ __forceinline__ void stockham_pass(cplx<T>* lds, int base, int stride
     if (b < nb) {
       const int j = tl + b * tpl;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = lds[lpad(base + (j + r * LR) * stride, psh)];
+      for (int r = 0; r < R; ++r) v[b][r] = lds[lpad<T>(base + (j + r * LR) * stride)];
     }
   }
   __syncthreads();
   const int twstep = L / (Ns * R);
+  // j = q * Ns + k; Ns is uniform, and a power of two for 2^n lengths
+  const bool pow2 = (Ns & (Ns - 1)) == 0;
+  const int nsh = __builtin_ctz(Ns);
 #pragma unroll
   for (int b = 0; b < NBMAX; ++b) {
     if (b < nb) {
       const int j = tl + b * tpl;
-      const int k = j % Ns;
+      int q, k;
+      if (pow2) {
+        q = j >> nsh;
+        k = j & (Ns - 1);
+      } else {
+        q = j / Ns;
+        k = j - q * Ns;
+      }
       if (Ns > 1) {
+        const int kt = k * twstep;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          cplx<T> w = tw[r * k * twstep];
+          cplx<T> w = tw[r * kt];
           v[b][r] = INV ? cmulc(v[b][r], w) : cmul(v[b][r], w);
         }
       }
       Dft<T, R, INV>::run(v[b]);
-      const int o = (j / Ns) * Ns * R + k;
+      const int o = q * Ns * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[lpad(base + (o + r * Ns) * stride, psh)] = v[b][r];
+      for (int r = 0; r < R; ++r) lds[lpad<T>(base + (o + r * Ns) * stride)] = v[b][r];
     }
   }
   __syncthreads();
+}
+
+// Butterflies of a pass (L/R of them, dealt round-robin to tpl threads) that
+// thread tl of the line owns.
+__device__ __forceinline__ int nbutterflies(int L, int R, int tl, int tpl) {
+  const int nbf = L / R;
+  const int full = nbf / tpl;  // uniform: no per-thread division
+  return full + (tl < nbf - full * tpl ? 1 : 0);
 }
 
 // One Stockham pass of radix R (R <= EMAX; radix 16 only exists in kernels
 // built for 16 or more register elements, so smaller builds stay lean).
 template <typename T, bool INV, int EMAX>
 __device__ __forceinline__ void pass_dispatch(int R, cplx<T>* lds, int base, int stride, int L, int Ns, int tl,
-                                              int tpl, int nb, const cplx<T>* __restrict__ tw, int psh) {
+                                              int tpl, int nb, const cplx<T>* __restrict__ tw) {
   switch (R) {
-    case 2: stockham_pass<T, 2, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw, psh); break;
-    case 3: stockham_pass<T, 3, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw, psh); break;
-    case 4: stockham_pass<T, 4, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw, psh); break;
-    case 5: stockham_pass<T, 5, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw, psh); break;
+    case 2: stockham_pass<T, 2, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
+    case 3: stockham_pass<T, 3, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
+    case 4: stockham_pass<T, 4, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
+    case 5: stockham_pass<T, 5, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
     case 16:
-      if constexpr (EMAX >= 16) stockham_pass<T, 16, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw, psh);
+      if constexpr (EMAX >= 16) stockham_pass<T, 16, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw);
       break;
-    default: stockham_pass<T, 8, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw, psh); break;
+    default: stockham_pass<T, 8, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
   }
 }
 
@@ -242,9 +256,8 @@ __device__ void fft_line(cplx<T>* lds, int base, int stride, const LinePlan& p, 
   int Ns = 1;
   for (int s = 0; s < p.npass; ++s) {
     const int R = p.radix[s];
-    const int nbf = p.L / R;  // butterflies of this pass, dealt round-robin to threads
-    const int nb = (active && tl < nbf) ? (nbf - tl + p.tpl - 1) / p.tpl : 0;
-    pass_dispatch<T, INV, EMAX>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw, p.psh);
+    const int nb = active ? nbutterflies(p.L, R, tl, p.tpl) : 0;
+    pass_dispatch<T, INV, EMAX>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw);
     Ns *= R;
   }
 }
@@ -256,10 +269,6 @@ __device__ void fft_line(cplx<T>* lds, int base, int stride, const LinePlan& p, 
 // first pass from registers and leaving the last pass in registers lets
 // kernels read inputs from / write outputs to HBM without an LDS round trip.
 
-__device__ __forceinline__ int nbutterflies(int L, int R, int tl, int tpl) {
-  const int nbf = L / R;
-  return tl < nbf ? (nbf - tl + tpl - 1) / tpl : 0;
-}
 
 // Element index held by register slot e for a pass of radix R.
 __device__ __forceinline__ int slot_element(int e, int R, int L, int tl, int tpl) {
@@ -268,7 +277,7 @@ __device__ __forceinline__ int slot_element(int e, int R, int L, int tl, int tpl
 
 template <typename T, int R, bool INV, int EMAX>
 __device__ __forceinline__ void first_pass_regs(const cplx<T>* v, cplx<T>* lds, int base, int stride, int L,
-                                                int tl, int tpl, int psh) {
+                                                int tl, int tpl) {
   constexpr int NBMAX = EMAX / R > 0 ? EMAX / R : 1;
   const int nb = nbutterflies(L, R, tl, tpl);
 #pragma unroll
@@ -280,14 +289,14 @@ __device__ __forceinline__ void first_pass_regs(const cplx<T>* v, cplx<T>* lds, 
       Dft<T, R, INV>::run(w);
       const int o = (tl + b * tpl) * R;  // Ns = 1
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[lpad(base + (o + r) * stride, psh)] = w[r];
+      for (int r = 0; r < R; ++r) lds[lpad<T>(base + (o + r) * stride)] = w[r];
     }
   }
 }
 
 template <typename T, int R, bool INV, int EMAX>
 __device__ __forceinline__ void last_pass_regs(cplx<T>* v, const cplx<T>* lds, int base, int stride, int L,
-                                               int tl, int tpl, const cplx<T>* __restrict__ tw, int psh) {
+                                               int tl, int tpl, const cplx<T>* __restrict__ tw) {
   constexpr int NBMAX = EMAX / R > 0 ? EMAX / R : 1;
   const int nb = nbutterflies(L, R, tl, tpl);
   const int LR = L / R;  // = Ns of the last pass
@@ -297,7 +306,7 @@ __device__ __forceinline__ void last_pass_regs(cplx<T>* v, const cplx<T>* lds, i
       const int j = tl + b * tpl;
       cplx<T> w[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) w[r] = lds[lpad(base + (j + r * LR) * stride, psh)];
+      for (int r = 0; r < R; ++r) w[r] = lds[lpad<T>(base + (j + r * LR) * stride)];
       if (LR > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
@@ -322,14 +331,14 @@ __device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const
   int s0 = 0, s1 = p.npass, Ns = 1;
   if (IN_REGS) {
     switch (p.radix[0]) {
-      case 2: first_pass_regs<T, 2, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, p.psh); break;
-      case 3: first_pass_regs<T, 3, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, p.psh); break;
-      case 4: first_pass_regs<T, 4, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, p.psh); break;
-      case 5: first_pass_regs<T, 5, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, p.psh); break;
+      case 2: first_pass_regs<T, 2, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
+      case 3: first_pass_regs<T, 3, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
+      case 4: first_pass_regs<T, 4, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
+      case 5: first_pass_regs<T, 5, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
       case 16:
-        if constexpr (EMAX >= 16) first_pass_regs<T, 16, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, p.psh);
+        if constexpr (EMAX >= 16) first_pass_regs<T, 16, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl);
         break;
-      default: first_pass_regs<T, 8, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, p.psh); break;
+      default: first_pass_regs<T, 8, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
     }
     __syncthreads();
     Ns = p.radix[0];
@@ -339,7 +348,7 @@ __device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const
   for (int s = s0; s < s1; ++s) {
     const int R = p.radix[s];
     const int nb = nbutterflies(p.L, R, tl, p.tpl);
-    pass_dispatch<T, INV, EMAX>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw, p.psh);
+    pass_dispatch<T, INV, EMAX>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw);
     Ns *= R;
   }
   if (OUT_REGS) {
@@ -349,18 +358,18 @@ __device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const
       const int nb = nbutterflies(p.L, R, tl, p.tpl);
 #pragma unroll
       for (int e = 0; e < EMAX; ++e)
-        if (e / R < nb) v[e] = lds[lpad(base + slot_element(e, R, p.L, tl, p.tpl) * stride, p.psh)];
+        if (e / R < nb) v[e] = lds[lpad<T>(base + slot_element(e, R, p.L, tl, p.tpl) * stride)];
       return;
     }
     switch (p.rlast) {
-      case 2: last_pass_regs<T, 2, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw, p.psh); break;
-      case 3: last_pass_regs<T, 3, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw, p.psh); break;
-      case 4: last_pass_regs<T, 4, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw, p.psh); break;
-      case 5: last_pass_regs<T, 5, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw, p.psh); break;
+      case 2: last_pass_regs<T, 2, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
+      case 3: last_pass_regs<T, 3, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
+      case 4: last_pass_regs<T, 4, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
+      case 5: last_pass_regs<T, 5, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
       case 16:
-        if constexpr (EMAX >= 16) last_pass_regs<T, 16, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw, p.psh);
+        if constexpr (EMAX >= 16) last_pass_regs<T, 16, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw);
         break;
-      default: last_pass_regs<T, 8, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw, p.psh); break;
+      default: last_pass_regs<T, 8, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
     }
   }
 }

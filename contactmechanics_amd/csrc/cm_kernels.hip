@@ -264,7 +264,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_r2c(Geom g, LinePlan lp
 #pragma unroll
   for (int i = 0; i < P::NS; ++i) acc[i] = red_identity(P::op(i));
   const int rb = r * N2;
-  auto L = [&](int n) -> cplx<T>& { return lds[lpad(rb + n, lp.psh)]; };
+  auto L = [&](int n) -> cplx<T>& { return lds[lpad<T>(rb + n)]; };
   // the producer feeds the first FFT pass directly (first-pass order)
   cplx<T> v[EMAX];
   {
@@ -313,7 +313,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_r2c(Geom g, LinePlan lp
   const int tot = g.R * N2;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int kk = i % g.Cb, rr = (i / g.Cb) % g.R, cb = i / RC;
-    S[((size_t)cb * g.Mx + blockIdx.x * g.R + rr) * g.Cb + kk] = lds[lpad(rr * N2 + cb * g.Cb + kk, lp.psh)];
+    S[((size_t)cb * g.Mx + blockIdx.x * g.R + rr) * g.Cb + kk] = lds[lpad<T>(rr * N2 + cb * g.Cb + kk)];
   }
   if constexpr (P::NS > 0) {
     if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
@@ -321,7 +321,17 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_r2c(Geom g, LinePlan lp
   }
 }
 
-// Column pass.  MODE 0: forward * G * inverse; 1: forward only; 2: inverse only.
+__device__ void tau_finalize(Scal* s, double sum_rt);
+
+// Reduction slots of the spectral tau pass (one sum).
+struct TauRed {
+  static constexpr int NS = 1;
+  __device__ static constexpr int op(int) { return RSUM; }
+};
+
+// Column pass.  MODE 0: forward * G * inverse; 1: forward only; 2: inverse
+// only; 3: forward only, then sum_k G(k) |T(k)|^2 over the whole spectrum and
+// the step length tau from it (no output).
 // skip: 0 never, 1 when the solve is done, 2 when done or in a mixing step.
 template <typename T, int EMAX, int MODE>
 __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp, LinePlan rv, const cplx<T>* __restrict__ tw,
@@ -339,6 +349,57 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
   const int Mx = g.Mx, Cb = g.Cb;
   const size_t base = (size_t)cb * Mx * Cb;
   const int n = Mx * Cb;
+  if (MODE == 3) {
+    // r = evaluate_disp(t) is only needed for sum_c r t (:287-292), and t
+    // vanishes off the contact set, so by Parseval
+    //   sum_x r(x) t(x) = sum_k G(k) |T(k)|^2
+    // over the full spectrum: columns 0 < ky < My/2 count twice (Hermitian
+    // half), the packed DC and Nyquist columns once each.
+    double acc = 0.0;
+    const int c = threadIdx.x % Cb, tl = threadIdx.x / Cb;
+    if (cb != 0) {
+      const int R0 = lp.radix[0], Rl = lp.rlast;
+      const int nb0 = nbutterflies(Mx, R0, tl, lp.tpl), nbl = nbutterflies(Mx, Rl, tl, lp.tpl);
+      cplx<T> v[EMAX];
+#pragma unroll
+      for (int e = 0; e < EMAX; ++e)
+        if (e / R0 < nb0) v[e] = S[base + (size_t)slot_element(e, R0, Mx, tl, lp.tpl) * Cb + c];
+      fft_line_x<T, false, EMAX, true, true>(v, lds, c, Cb, lp, tl, tw);
+      const T* Gk = G + (size_t)(cb * Cb + c) * Mx;
+#pragma unroll
+      for (int e = 0; e < EMAX; ++e)
+        if (e / Rl < nbl) {
+          const double a = (double)v[e].x, b = (double)v[e].y;
+          acc += 2.0 * (double)Gk[slot_element(e, Rl, Mx, tl, lp.tpl)] * (a * a + b * b);
+        }
+    } else {
+      for (int i = threadIdx.x; i < n; i += blockDim.x) lds[lpad<T>(i)] = S[base + i];
+      __syncthreads();
+      fft_line<T, false, EMAX>(lds, c, Cb, lp, tl, true, tw);
+      if (c != 0) {
+        const T* Gk = G + (size_t)c * Mx;
+        for (int x = tl; x < Mx; x += lp.tpl) {
+          const cplx<T> q = lds[lpad<T>(x * Cb + c)];
+          acc += 2.0 * (double)Gk[x] * ((double)q.x * q.x + (double)q.y * q.y);
+        }
+      } else {
+        // Q = A + iB: A(kx) = (Q(kx) + conj Q(-kx)) / 2 (ky = 0),
+        // B(kx) = (Q(kx) - conj Q(-kx)) / 2i (ky = My/2)
+        const T* G0 = G;
+        const T* GN = G + (size_t)g.N2 * Mx;
+        for (int x = tl; x < Mx; x += lp.tpl) {
+          const int xm = (Mx - x) % Mx;
+          const cplx<T> q = lds[lpad<T>(x * Cb)], qm = lds[lpad<T>(xm * Cb)];
+          const double ar = 0.5 * ((double)q.x + qm.x), ai = 0.5 * ((double)q.y - qm.y);
+          const double br = 0.5 * ((double)q.x - qm.x), bi = 0.5 * ((double)q.y + qm.y);
+          acc += (double)G0[x] * (ar * ar + ai * ai) + (double)GN[x] * (br * br + bi * bi);
+        }
+      }
+    }
+    if (reduce_blocks<TauRed>(&acc, in.part, in.ctr + 4 * kCtrSlot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
+      tau_finalize(in.sc, acc);
+    return;
+  }
   if (MODE == 0 && cb != 0) {
     // every column is an ordinary one: HBM -> first forward pass, last
     // forward pass -> x G in registers -> first inverse pass (same radix, same
@@ -364,7 +425,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
       if (e / R0 < nb0) S[base + (size_t)slot_element(e, R0, Mx, tl, lp.tpl) * Cb + c] = v[e];
     return;
   }
-  for (int i = threadIdx.x; i < n; i += blockDim.x) lds[lpad(i, lp.psh)] = S[base + i];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) lds[lpad<T>(i)] = S[base + i];
   __syncthreads();
   const int c = threadIdx.x % Cb, tl = threadIdx.x / Cb;
   if (MODE != 2) fft_line<T, false, EMAX>(lds, c, Cb, lp, tl, true, tw);
@@ -373,7 +434,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
     if (ky != 0) {
       const T* Gk = G + (size_t)ky * Mx;
       for (int x = tl; x < Mx; x += lp.tpl) {
-        cplx<T>& e = lds[lpad(x * Cb + c, lp.psh)];
+        cplx<T>& e = lds[lpad<T>(x * Cb + c)];
         e = e * Gk[x];
       }
     } else {
@@ -383,18 +444,18 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
       const T* GN = G + (size_t)g.N2 * Mx;
       for (int x = tl; x <= Mx / 2; x += lp.tpl) {
         const int xm = (Mx - x) % Mx;
-        cplx<T> q = lds[lpad(x * Cb, lp.psh)], qm = lds[lpad(xm * Cb, lp.psh)];
+        cplx<T> q = lds[lpad<T>(x * Cb)], qm = lds[lpad<T>(xm * Cb)];
         const T a0 = G0[x], aN = GN[x], b0 = G0[xm], bN = GN[xm];
         cplx<T> nq = q * (T(0.5) * (a0 + aN)) + conj(qm) * (T(0.5) * (a0 - aN));
         cplx<T> nqm = qm * (T(0.5) * (b0 + bN)) + conj(q) * (T(0.5) * (b0 - bN));
-        lds[lpad(x * Cb, lp.psh)] = nq;
-        if (xm != x) lds[lpad(xm * Cb, lp.psh)] = nqm;
+        lds[lpad<T>(x * Cb)] = nq;
+        if (xm != x) lds[lpad<T>(xm * Cb)] = nqm;
       }
     }
     __syncthreads();
   }
   if (MODE != 1) fft_line<T, true, EMAX>(lds, c, Cb, lp, tl, true, tw);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) S[base + i] = lds[lpad(i, lp.psh)];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) S[base + i] = lds[lpad<T>(i)];
 }
 
 // Row pass: packed half spectrum -> real rows (inverse, unnormalised; the
@@ -413,13 +474,13 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_c2r(Geom g, LinePlan lp
   const int tot = g.R * N2;
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int kk = i % g.Cb, rr = (i / g.Cb) % g.R, cb = i / RC;
-    lds[lpad(rr * N2 + cb * g.Cb + kk, lp.psh)] = S[((size_t)cb * g.Mx + blockIdx.x * g.R + rr) * g.Cb + kk];
+    lds[lpad<T>(rr * N2 + cb * g.Cb + kk)] = S[((size_t)cb * g.Mx + blockIdx.x * g.R + rr) * g.Cb + kk];
   }
   __syncthreads();
   const int r = threadIdx.x / lp.tpl, tl = threadIdx.x - r * lp.tpl;
   const int row = blockIdx.x * g.R + r;
   const int rb = r * N2;
-  auto L = [&](int n) -> cplx<T>& { return lds[lpad(rb + n, lp.psh)]; };
+  auto L = [&](int n) -> cplx<T>& { return lds[lpad<T>(rb + n)]; };
   for (int k = tl; k <= N2 / 2; k += lp.tpl) {
     if (k == 0) {
       cplx<T> z = L(0);  // (X[0], X[N2])
@@ -795,40 +856,18 @@ struct ProdDir {
   __device__ void finalize(double* a) const { in.sc->Scgt = a[0]; }
 };
 
-// r = evaluate_disp(t) consumed on the fly: sum c r t and the step length tau
-// (:284-294).
-template <typename T>
-struct ConsTau {
-  static constexpr int NS = 1;
-  __device__ static constexpr int op(int) { return RSUM; }
-  struct Rec {
-    P2<T> t;
-    P2<uint8_t> cm;
-  };
-  const Inst<T>& in;
-  __device__ ConsTau(const Inst<T>& i) : in(i) {}
-  __device__ bool skip() const { return in.sc->done || in.sc->mode == MODE_MIX; }
-  __device__ Rec fetch(const Geom& g, int row, int col) const {
-    const size_t p = (size_t)row * g.My + col;
-    return Rec{ld2(in.t + p), ldm(in.cm + p)};
+// Step length from sum_c r t (:284-294); r = evaluate_disp(t).
+__device__ void tau_finalize(Scal* s, double sum_rt) {
+  s->nfev += 1;
+  s->tau = 0.0;
+  if (s->A_cg > 0) {
+    const double x = -sum_rt;
+    if (x > 0.0)
+      s->tau = s->Scgt / x;
+    else
+      s->G = 0.0;
   }
-  __device__ void take(const Geom&, int, int, const Rec& r, T a, T b, double* acc) const {
-    if (r.cm.a & kNext) acc[0] += (double)a * (double)r.t.a;
-    if (r.cm.b & kNext) acc[0] += (double)b * (double)r.t.b;
-  }
-  __device__ void finalize(double* a) const {
-    Scal* s = in.sc;
-    s->nfev += 1;
-    s->tau = 0.0;
-    if (s->A_cg > 0) {
-      const double x = -a[0];
-      if (x > 0.0)
-        s->tau = s->Scgt / x;
-      else
-        s->G = 0.0;
-    }
-  }
-};
+}
 
 // Per-pixel force after the PK or mixing update, before projection
 // (:296 and :322-323).  Shared by phase B and the second transform.
@@ -1191,7 +1230,6 @@ bool make_line_plan(int L, int max_tpl, int emax, LinePlan& best) {
   if (L < 2) return false;
   LinePlan lp{};
   lp.L = L;
-  lp.psh = kNoPad;  // set by the shape chooser
   int rem = L, np = 0;
   while (rem > 1) {
     int pick = 0;
@@ -1301,23 +1339,12 @@ bool choose_shape(FftExec& ex, const Shape& sh) {
   const size_t cs = sizeof(cplx<T>);
   constexpr size_t kLdsAuto = 80 * 1024;   // automatic shapes: two blocks per CU (160 KiB LDS)
   constexpr size_t kLdsMax = 160 * 1024;   // largest block allocation on gfx950
-  const int pad = kPadShift<T>;
   // register elements per thread: 8 (lean kernels, four or more waves per
   // SIMD), more when the line does not fit a block otherwise
   auto plan_len = [&](int L, int e, LinePlan& lp) {
     if (e) return make_line_plan(L, fft_block_max(e), e, lp);
     return make_line_plan(L, fft_block_max(8), 8, lp) || make_line_plan(L, fft_block_max(16), 16, lp) ||
            make_line_plan(L, fft_block_max(32), 32, lp);
-  };
-  // padding shift for a slab of n values: padded when that leaves room for
-  // two blocks per CU, else unpadded when that does, else whichever fits one
-  // block per CU (or nothing fits: -1)
-  auto slab_shift = [&](size_t n) {
-    for (size_t cap : {kLdsAuto, kLdsMax}) {
-      if (lds_bytes<T>(n, pad) <= cap) return pad;
-      if (lds_bytes<T>(n, kNoPad) <= cap) return kNoPad;
-    }
-    return -1;
   };
   if (!plan_len(g.N2, sh.erow, ex.row) || !plan_len(Mx, sh.ecol, ex.col)) return false;
   const int row_max = fft_block_max(emax_bucket(ex.row.E)), col_max = fft_block_max(emax_bucket(ex.col.E));
@@ -1329,29 +1356,25 @@ bool choose_shape(FftExec& ex, const Shape& sh) {
     if (g.N2 % Cb || ex.col.tpl * Cb > col_max) return false;
   } else {
     const int cb_target = (int)(64 / cs) > 0 ? (int)(64 / cs) : 1;
-    while (Cb * 2 <= cb_target && g.N2 % (Cb * 2) == 0 && lds_bytes<T>((size_t)Mx * Cb * 2, pad) <= kLdsAuto &&
+    while (Cb * 2 <= cb_target && g.N2 % (Cb * 2) == 0 && lds_bytes<T>((size_t)Mx * Cb * 2) <= kLdsAuto &&
            ex.col.tpl * Cb * 2 <= col_max)
       Cb *= 2;
   }
-  const int csh = slab_shift((size_t)Mx * Cb);
-  if (csh < 0) return false;
+  if (lds_bytes<T>((size_t)Mx * Cb) > kLdsMax) return false;
   int R = 1;
   if (sh.R) {
     R = sh.R;
     if (Mx % R || ex.row.tpl * R > row_max) return false;
   } else {
     while (Mx % (R * 2) == 0 && ex.row.tpl * R * 2 <= row_max &&
-           lds_bytes<T>((size_t)R * 2 * g.N2, pad) <= kLdsAuto && (size_t)R * Cb * cs < 128)
+           lds_bytes<T>((size_t)R * 2 * g.N2) <= kLdsAuto && (size_t)R * Cb * cs < 128)
       R *= 2;
   }
-  const int rsh = slab_shift((size_t)R * g.N2);
-  if (rsh < 0) return false;
+  if (lds_bytes<T>((size_t)R * g.N2) > kLdsMax) return false;
   g.Cb = Cb;
   g.R = R;
-  ex.row.psh = rsh;
-  ex.col.psh = csh;
-  ex.lds_row = lds_bytes<T>((size_t)R * g.N2, rsh);
-  ex.lds_col = lds_bytes<T>((size_t)Mx * Cb, csh);
+  ex.lds_row = lds_bytes<T>((size_t)R * g.N2);
+  ex.lds_col = lds_bytes<T>((size_t)Mx * Cb);
   ex.emax_row = emax_bucket(ex.row.E);
   ex.emax_col = emax_bucket(ex.col.E);
   ex.col_rev = ex.col;
@@ -1698,7 +1721,11 @@ size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
 WsLayout ws_layout(const cm_plan* P, int B) {
   const size_t w = P->prec == CM_FP64 ? 8 : 4;
   const size_t M = (size_t)P->Mx * P->My;
-  const size_t nparts = std::max<size_t>((size_t)P->nblk_elem, (size_t)(P->Mx / P->ex.g.R)) * 8;
+  // partials: up to 8 slots per block of any reducing launch (element-wise,
+  // row passes, column pass)
+  const size_t nblk_max = std::max<size_t>({(size_t)P->nblk_elem, (size_t)(P->Mx / P->ex.g.R),
+                                            (size_t)(P->ex.g.N2 / P->ex.g.Cb)});
+  const size_t nparts = nblk_max * 8;
   WsLayout L{};
   L.st_t = al256(M * w);
   L.st_S = al256((M / 2) * 2 * w);
@@ -1748,6 +1775,7 @@ void fill_stats(const Scal& s, cm_ccg_stats* o) {
 // device between calls, and the captured iteration graphs.
 struct cm_session {
   cm_plan* P = nullptr;
+  FftExec ex;  // the plan's launch shapes when the session began (the workspace is sized for them)
   int B = 0;
   hipStream_t st = nullptr;
   void* ws = nullptr;
@@ -1769,7 +1797,7 @@ namespace {
 
 template <typename T>
 void enqueue_iteration(cm_session* S) {
-  const FftExec& ex = S->P->ex;
+  const FftExec& ex = S->ex;
   InstArr<T> ia{};
   ia.arr = (const Inst<T>*)((char*)S->ws + S->L.inst);
   const int B = S->B, nbe = S->P->nblk_elem;
@@ -1778,8 +1806,7 @@ void enqueue_iteration(cm_session* S) {
   // one runs at set-up
   if (S->any_bisect) hipLaunchKernelGGL((k_bisect<T>), dim3(B), dim3(1024), 0, st, ex.g, ia);
   launch_r2c<T, ProdDir<T>>(ex, ia, B, 3, st);
-  launch_col<T, 0>(ex, ia, B, 0, 2, st);
-  launch_c2r<T, ConsTau<T>>(ex, ia, B, 4, st);
+  launch_col<T, 3>(ex, ia, B, 0, 2, st);  // tau from the spectrum: no inverse pass
   launch_elem<T, ElemPhaseB<T>>(ex, nbe, ia, B, 5, st);
   launch_r2c<T, ProdForce<T>>(ex, ia, B, 6, st);
   launch_col<T, 0>(ex, ia, B, 0, 1, st);
@@ -1796,6 +1823,7 @@ int session_begin(cm_plan* P, int B, const cm_ccg_params* prm, const double* con
       return fail(ctx, CM_E_ARG, "all instances of a batch must share init_mode");
   cm_session* S = new cm_session();
   S->P = P;
+  S->ex = P->ex;
   S->B = B;
   S->st = stream ? (hipStream_t)stream : ctx->stream;
   S->L = ws_layout(P, B);
@@ -1898,7 +1926,7 @@ int session_begin(cm_plan* P, int B, const cm_ccg_params* prm, const double* con
   ia.arr = dinst;
   InstArr<T> ia0{};
   ia0.arr = dinst0;
-  const FftExec& ex = P->ex;
+  const FftExec& ex = S->ex;
   const int nbe = P->nblk_elem;
   // ---- set-up (:179-219)
   launch_elem<T, ElemInit<T>>(ex, nbe, ia, B, 0, st);
@@ -1960,7 +1988,7 @@ int session_step(cm_session* S, int n) {
 
 template <typename T>
 int session_finish(cm_session* S) {
-  const FftExec& ex = S->P->ex;
+  const FftExec& ex = S->ex;
   InstArr<T> ia{};
   ia.arr = (const Inst<T>*)((char*)S->ws + S->L.inst);
   launch_elem<T, ElemEnergy<T>>(ex, S->P->nblk_elem, ia, S->B, 0, S->st);
@@ -1971,7 +1999,7 @@ int session_finish(cm_session* S) {
 template <typename T>
 int session_profile(cm_session* S, int n, float* us) {
   cm_ctx* ctx = S->P->ctx;
-  const FftExec& ex = S->P->ex;
+  const FftExec& ex = S->ex;
   InstArr<T> ia{};
   ia.arr = (const Inst<T>*)((char*)S->ws + S->L.inst);
   const int B = S->B, nbe = S->P->nblk_elem;
@@ -1989,9 +2017,9 @@ int session_profile(cm_session* S, int n, float* us) {
     HIPCHK(ctx, hipEventRecord(ev[k++], st));
     launch_r2c<T, ProdDir<T>>(ex, ia, B, 3, st);
     HIPCHK(ctx, hipEventRecord(ev[k++], st));
-    launch_col<T, 0>(ex, ia, B, 0, 2, st);
+    launch_col<T, 3>(ex, ia, B, 0, 2, st);
     HIPCHK(ctx, hipEventRecord(ev[k++], st));
-    launch_c2r<T, ConsTau<T>>(ex, ia, B, 4, st);
+    // (tau comes from the spectrum in the column pass: no c2r pass)
     HIPCHK(ctx, hipEventRecord(ev[k++], st));
     launch_elem<T, ElemPhaseB<T>>(ex, nbe, ia, B, 5, st);
     HIPCHK(ctx, hipEventRecord(ev[k++], st));

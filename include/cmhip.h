@@ -178,10 +178,13 @@ int cm_ccg_step(cm_session* s, int n_iters);
 int cm_ccg_poll(cm_session* s, cm_ccg_stats* stats, int* all_done);
 int cm_ccg_finish(cm_session* s, cm_ccg_stats* stats);
 int cm_ccg_end(cm_session* s);
-/* Run n iterations launch by launch with HIP events between the 9 kernels of
- * an iteration; kernel_us[9] receives the mean duration of each (phase A,
- * bisection, direction r2c, column, tau c2r, phase B, force r2c, column,
- * displacement c2r).  Measurement hook, no reference counterpart. */
+/* Run n iterations launch by launch with HIP events between the kernels of
+ * an iteration; kernel_us[9] receives the mean time of each slot (phase A,
+ * bisection, direction r2c, direction column pass with the step length,
+ * tau c2r, phase B, force r2c, column, displacement c2r).  Phase A is fused
+ * into the displacement c2r and the step length into the direction column
+ * pass, so slots 0 and 4 measure only the gap between two launches.
+ * Measurement hook, no reference counterpart. */
 int cm_ccg_profile(cm_session* s, int n_iters, float* kernel_us);
 
 #ifdef __cplusplus
