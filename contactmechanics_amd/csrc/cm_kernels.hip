@@ -243,10 +243,11 @@ __device__ __forceinline__ HPair ldh(const Geom& g, const T* h, const uint8_t* s
   return r;
 }
 
-// Largest block of an FFT kernel: 512 threads for the lean 8-element builds
-// (about 120 VGPRs: two such blocks fill a CU at four waves per SIMD), 256
-// for the 16/32-element builds.
-__host__ __device__ constexpr int fft_block_max(int emax) { return emax == 8 ? 512 : 256; }
+// Largest block of an FFT kernel: 512 threads.  The lean 8-element builds
+// (about 120 VGPRs) fit two such blocks per CU at four waves per SIMD; the
+// 16/32-element builds (up to 256 VGPRs) one, and they serve the long lines
+// (up to 8192 points, e.g. 8000 = 16*4*5^3) that need more than 256 threads.
+__host__ __device__ constexpr int fft_block_max(int) { return 512; }
 
 // Row pass: real rows -> packed half spectrum (forward, unnormalised).
 template <typename T, int EMAX, class P>
