@@ -189,9 +189,11 @@ __device__ bool reduce_blocks(double* acc, double* part, unsigned* ctr, int nblo
 // the loads are in flight together instead of waiting behind stores.
 constexpr int kPairs = 4;
 
-template <typename T> struct Vec2;
-template <> struct Vec2<float> { using type = float2; };
-template <> struct Vec2<double> { using type = double2; };
+// Field pairs are accessed as aligned 2-vectors in the global address space
+// (instance-table pointers would otherwise compile to flat accesses, which the
+// compiler also counts against LDS traffic).
+template <typename T>
+using Gv2 = T __attribute__((ext_vector_type(2)));
 
 template <typename T>
 struct P2 {
@@ -199,22 +201,22 @@ struct P2 {
 };
 template <typename T>
 __device__ __forceinline__ P2<T> ld2(const T* p) {
-  const typename Vec2<T>::type v = *reinterpret_cast<const typename Vec2<T>::type*>(p);
-  return P2<T>{v.x, v.y};
+  const Gv2<T> v = *(const __attribute__((address_space(1))) Gv2<T>*)p;
+  return P2<T>{v[0], v[1]};
 }
 template <typename T>
 __device__ __forceinline__ void st2(T* p, T a, T b) {
-  typename Vec2<T>::type v;
-  v.x = a;
-  v.y = b;
-  *reinterpret_cast<typename Vec2<T>::type*>(p) = v;
+  Gv2<T> v;
+  v[0] = a;
+  v[1] = b;
+  *(__attribute__((address_space(1))) Gv2<T>*)p = v;
 }
 __device__ __forceinline__ P2<uint8_t> ldm(const uint8_t* p) {
-  const uint16_t v = *reinterpret_cast<const uint16_t*>(p);
+  const uint16_t v = *(const __attribute__((address_space(1))) uint16_t*)p;
   return P2<uint8_t>{(uint8_t)(v & 0xff), (uint8_t)(v >> 8)};
 }
 __device__ __forceinline__ void stm(uint8_t* p, uint8_t a, uint8_t b) {
-  *reinterpret_cast<uint16_t*>(p) = (uint16_t)(a | (b << 8));
+  *(__attribute__((address_space(1))) uint16_t*)p = (uint16_t)(a | (b << 8));
 }
 
 // Heights and validity of the pair (row, col), (row, col+1).
@@ -242,6 +244,79 @@ __device__ __forceinline__ HPair ldh(const Geom& g, const T* h, const uint8_t* s
   }
   return r;
 }
+
+// Global-memory access to complex arrays that reach kernels through instance
+// tables: plain (generic) pointers compile to flat loads, which the compiler
+// must also count against LDS traffic, so every LDS access would wait for
+// them.  The views address the real and imaginary parts as scalars.
+template <typename T>
+struct GView {
+  const __attribute__((address_space(1))) T* p;
+  __device__ explicit GView(const cplx<T>* q) : p((const __attribute__((address_space(1))) T*)q) {}
+  __device__ __forceinline__ cplx<T> operator[](size_t i) const { return mk(p[2 * i], p[2 * i + 1]); }
+};
+template <typename T>
+struct GViewW {
+  __attribute__((address_space(1))) T* p;
+  __device__ explicit GViewW(cplx<T>* q) : p((__attribute__((address_space(1))) T*)q) {}
+  __device__ __forceinline__ void put(size_t i, cplx<T> v) const {
+    p[2 * i] = v.x;
+    p[2 * i + 1] = v.y;
+  }
+};
+
+// Slab copies between HBM and LDS with kCopyU loads (or stores) in flight per
+// thread: a plain strided loop waits for each load before the next.
+constexpr int kCopyU = 8;
+
+// Contiguous slab: lds[pad(i)] <- S[base + i], i < n.
+template <typename T>
+__device__ __forceinline__ void slab_load(cplx<T>* lds, const cplx<T>* Sp, size_t base, int n) {
+  const GView<T> S(Sp);
+  for (int i0 = threadIdx.x; i0 < n; i0 += kCopyU * (int)blockDim.x) {
+    // out-of-range slots repeat the last element (an identical, harmless
+    // second write): no branches, so the values stay in registers (as
+    // separate real/imaginary arrays, which the compiler keeps out of
+    // private memory) and all loads are in flight together
+    T tx[kCopyU], ty[kCopyU];
+#pragma unroll
+    for (int k = 0; k < kCopyU; ++k) {
+      const cplx<T> z = S[base + min(i0 + k * (int)blockDim.x, n - 1)];
+      tx[k] = z.x;
+      ty[k] = z.y;
+    }
+#pragma unroll
+    for (int k = 0; k < kCopyU; ++k) lds[lpad<T>(min(i0 + k * (int)blockDim.x, n - 1))] = mk(tx[k], ty[k]);
+  }
+}
+template <typename T>
+__device__ __forceinline__ void slab_store(cplx<T>* Sp, const cplx<T>* lds, size_t base, int n) {
+  const GViewW<T> S(Sp);
+  for (int i0 = threadIdx.x; i0 < n; i0 += kCopyU * (int)blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < kCopyU; ++k) {
+      const int i = i0 + k * (int)blockDim.x;
+      if (i < n) S.put(base + i, lds[lpad<T>(i)]);
+    }
+  }
+}
+
+// Row-block <-> column-blocked spectrum.  Element i of the block's R rows x N2
+// values: kk = i % Cb, rr = (i / Cb) % R, cb = i / (R Cb) (Cb, R powers of two);
+// LDS holds row rr at rr * N2, the spectrum element (x = row, ky = cb Cb + kk).
+struct RowBlockMap {
+  int cbs, rs, Cb, R, N2, Mx, row0;
+  __device__ RowBlockMap(const Geom& g, int blk)
+      : cbs(__builtin_ctz(g.Cb)), rs(__builtin_ctz(g.R)), Cb(g.Cb), R(g.R), N2(g.N2), Mx(g.Mx), row0(blk * g.R) {}
+  __device__ __forceinline__ size_t spec(int i) const {
+    const int kk = i & (Cb - 1), rr = (i >> cbs) & (R - 1), cb = i >> (cbs + rs);
+    return ((size_t)cb * Mx + row0 + rr) * Cb + kk;
+  }
+  __device__ __forceinline__ int lds(int i) const {
+    const int kk = i & (Cb - 1), rr = (i >> cbs) & (R - 1), cb = i >> (cbs + rs);
+    return rr * N2 + cb * Cb + kk;
+  }
+};
 
 // Largest block of an FFT kernel: 512 threads.  The lean 8-element builds
 // (about 120 VGPRs) fit two such blocks per CU at four waves per SIMD; the
@@ -310,11 +385,15 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_r2c(Geom g, LinePlan lp
   }
   __syncthreads();
   cplx<T>* S = in.S;
-  const int RC = g.R * g.Cb;
   const int tot = g.R * N2;
-  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-    const int kk = i % g.Cb, rr = (i / g.Cb) % g.R, cb = i / RC;
-    S[((size_t)cb * g.Mx + blockIdx.x * g.R + rr) * g.Cb + kk] = lds[lpad<T>(rr * N2 + cb * g.Cb + kk)];
+  const RowBlockMap map(g, blockIdx.x);
+  const GViewW<T> Sg(S);
+  for (int i0 = threadIdx.x; i0 < tot; i0 += kCopyU * (int)blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < kCopyU; ++k) {
+      const int i = i0 + k * (int)blockDim.x;
+      if (i < tot) Sg.put(map.spec(i), lds[lpad<T>(map.lds(i))]);
+    }
   }
   if constexpr (P::NS > 0) {
     if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
@@ -362,11 +441,12 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
       const int R0 = lp.radix[0], Rl = lp.rlast;
       const int nb0 = nbutterflies(Mx, R0, tl, lp.tpl), nbl = nbutterflies(Mx, Rl, tl, lp.tpl);
       cplx<T> v[EMAX];
+      const GView<T> Sg(S);
 #pragma unroll
       for (int e = 0; e < EMAX; ++e)
-        if (e / R0 < nb0) v[e] = S[base + (size_t)slot_element(e, R0, Mx, tl, lp.tpl) * Cb + c];
+        if (e / R0 < nb0) v[e] = Sg[base + (size_t)slot_element(e, R0, Mx, tl, lp.tpl) * Cb + c];
       fft_line_x<T, false, EMAX, true, true>(v, lds, c, Cb, lp, tl, tw);
-      const T* Gk = G + (size_t)(cb * Cb + c) * Mx;
+      const auto Gk = (const __attribute__((address_space(1))) T*)(G + (size_t)(cb * Cb + c) * Mx);
 #pragma unroll
       for (int e = 0; e < EMAX; ++e)
         if (e / Rl < nbl) {
@@ -374,7 +454,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
           acc += 2.0 * (double)Gk[slot_element(e, Rl, Mx, tl, lp.tpl)] * (a * a + b * b);
         }
     } else {
-      for (int i = threadIdx.x; i < n; i += blockDim.x) lds[lpad<T>(i)] = S[base + i];
+      slab_load(lds, S, base, n);
       __syncthreads();
       fft_line<T, false, EMAX>(lds, c, Cb, lp, tl, true, tw);
       if (c != 0) {
@@ -409,24 +489,26 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
     const int R0 = lp.radix[0], Rl = lp.rlast;
     const int nb0 = nbutterflies(Mx, R0, tl, lp.tpl), nbl = nbutterflies(Mx, Rl, tl, lp.tpl);
     cplx<T> v[EMAX];
+    const GView<T> Sg(S);
 #pragma unroll
     for (int e = 0; e < EMAX; ++e)
-      if (e / R0 < nb0) v[e] = S[base + (size_t)slot_element(e, R0, Mx, tl, lp.tpl) * Cb + c];
+      if (e / R0 < nb0) v[e] = Sg[base + (size_t)slot_element(e, R0, Mx, tl, lp.tpl) * Cb + c];
     fft_line_x<T, false, EMAX, true, true>(v, lds, c, Cb, lp, tl, tw);
     __syncthreads();
-    const T* Gk = G + (size_t)(cb * Cb + c) * Mx;
+    const auto Gk = (const __attribute__((address_space(1))) T*)(G + (size_t)(cb * Cb + c) * Mx);
 #pragma unroll
     for (int e = 0; e < EMAX; ++e)
       if (e / Rl < nbl) v[e] = v[e] * Gk[slot_element(e, Rl, Mx, tl, lp.tpl)];
     // rv: the plan with its radices reversed, so the inverse transform's first
     // pass takes the forward transform's last-pass register slots as they are
     fft_line_x<T, true, EMAX, true, true>(v, lds, c, Cb, rv, tl, tw);
+    const GViewW<T> Sw(S);
 #pragma unroll
     for (int e = 0; e < EMAX; ++e)
-      if (e / R0 < nb0) S[base + (size_t)slot_element(e, R0, Mx, tl, lp.tpl) * Cb + c] = v[e];
+      if (e / R0 < nb0) Sw.put(base + (size_t)slot_element(e, R0, Mx, tl, lp.tpl) * Cb + c, v[e]);
     return;
   }
-  for (int i = threadIdx.x; i < n; i += blockDim.x) lds[lpad<T>(i)] = S[base + i];
+  slab_load(lds, S, base, n);
   __syncthreads();
   const int c = threadIdx.x % Cb, tl = threadIdx.x / Cb;
   if (MODE != 2) fft_line<T, false, EMAX>(lds, c, Cb, lp, tl, true, tw);
@@ -456,7 +538,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
     __syncthreads();
   }
   if (MODE != 1) fft_line<T, true, EMAX>(lds, c, Cb, lp, tl, true, tw);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) S[base + i] = lds[lpad<T>(i)];
+  slab_store(S, lds, base, n);
 }
 
 // Row pass: packed half spectrum -> real rows (inverse, unnormalised; the
@@ -471,11 +553,20 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_c2r(Geom g, LinePlan lp
   if (p.skip()) return;
   const int N2 = g.N2;
   const cplx<T>* S = in.S;
-  const int RC = g.R * g.Cb;
   const int tot = g.R * N2;
-  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-    const int kk = i % g.Cb, rr = (i / g.Cb) % g.R, cb = i / RC;
-    lds[lpad<T>(rr * N2 + cb * g.Cb + kk)] = S[((size_t)cb * g.Mx + blockIdx.x * g.R + rr) * g.Cb + kk];
+  const RowBlockMap map(g, blockIdx.x);
+  const GView<T> Sg(S);
+  for (int i0 = threadIdx.x; i0 < tot; i0 += kCopyU * (int)blockDim.x) {
+    // (out-of-range slots repeat the last element, as in slab_load)
+    T tx[kCopyU], ty[kCopyU];
+#pragma unroll
+    for (int k = 0; k < kCopyU; ++k) {
+      const cplx<T> z = Sg[map.spec(min(i0 + k * (int)blockDim.x, tot - 1))];
+      tx[k] = z.x;
+      ty[k] = z.y;
+    }
+#pragma unroll
+    for (int k = 0; k < kCopyU; ++k) lds[lpad<T>(map.lds(min(i0 + k * (int)blockDim.x, tot - 1)))] = mk(tx[k], ty[k]);
   }
   __syncthreads();
   const int r = threadIdx.x / lp.tpl, tl = threadIdx.x - r * lp.tpl;
@@ -1354,7 +1445,7 @@ bool choose_shape(FftExec& ex, const Shape& sh) {
   int Cb = 1;
   if (sh.Cb) {
     Cb = sh.Cb;
-    if (g.N2 % Cb || ex.col.tpl * Cb > col_max) return false;
+    if ((Cb & (Cb - 1)) || g.N2 % Cb || ex.col.tpl * Cb > col_max) return false;
   } else {
     const int cb_target = (int)(64 / cs) > 0 ? (int)(64 / cs) : 1;
     while (Cb * 2 <= cb_target && g.N2 % (Cb * 2) == 0 && lds_bytes<T>((size_t)Mx * Cb * 2) <= kLdsAuto &&
@@ -1365,7 +1456,7 @@ bool choose_shape(FftExec& ex, const Shape& sh) {
   int R = 1;
   if (sh.R) {
     R = sh.R;
-    if (Mx % R || ex.row.tpl * R > row_max) return false;
+    if ((R & (R - 1)) || Mx % R || ex.row.tpl * R > row_max) return false;
   } else {
     while (Mx % (R * 2) == 0 && ex.row.tpl * R * 2 <= row_max &&
            lds_bytes<T>((size_t)R * 2 * g.N2) <= kLdsAuto && (size_t)R * Cb * cs < 128)
