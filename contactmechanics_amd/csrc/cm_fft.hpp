@@ -374,6 +374,20 @@ __device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const
   }
 }
 
+// All passes but the last, from LDS (natural order) to LDS; the caller runs
+// the last pass itself (last_pass_regs with its radix as a template argument,
+// so the register slots it fills are compile-time indices).
+template <typename T, bool INV, int EMAX>
+__device__ __forceinline__ void fft_line_to_last(cplx<T>* lds, int base, int stride, const LinePlan& p, int tl,
+                                 const cplx<T>* __restrict__ tw) {
+  int Ns = 1;
+  for (int s = 0; s + 1 < p.npass; ++s) {
+    const int R = p.radix[s];
+    pass_dispatch<T, INV, EMAX>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nbutterflies(p.L, R, tl, p.tpl), tw);
+    Ns *= R;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Reductions in FP64.  A reduction record holds NS slots; OPS[i] is the op.
 enum RedOp : int { RSUM = 0, RMAX = 1, RMIN = 2 };

@@ -121,7 +121,7 @@ constexpr int kCtrSlot = kCtrLine * (kTicketGroups + 1); // counters of one redu
 // them to complete (s_waitcnt vmcnt(0)) before it takes its ticket, and only
 // the last block invalidates its L2 (agent-scope acquire) before reading them.
 template <class P>
-__device__ bool reduce_blocks(double* acc, double* part, unsigned* ctr, int nblocks, int bid) {
+__device__ __forceinline__ bool reduce_blocks(double* acc, double* part, unsigned* ctr, int nblocks, int bid) {
   constexpr int NS = P::NS;
   __shared__ double scr[8 * 16];
   __shared__ int s_last;
@@ -318,6 +318,17 @@ struct RowBlockMap {
   }
 };
 
+// The last block's scalar logic gets a copy of the reduced slots, so the
+// accumulator array itself never has its address taken and stays in registers
+// (the functors' finalize is not necessarily inlined).
+template <class P>
+__device__ __forceinline__ void finalize_copy(const P& p, const double* acc) {
+  double fin[P::NS > 0 ? P::NS : 1];
+#pragma unroll
+  for (int i = 0; i < P::NS; ++i) fin[i] = acc[i];
+  p.finalize(fin);
+}
+
 // Largest block of an FFT kernel: 512 threads.  The lean 8-element builds
 // (about 120 VGPRs) fit two such blocks per CU at four waves per SIMD; the
 // 16/32-element builds (up to 256 VGPRs) one, and they serve the long lines
@@ -397,7 +408,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_r2c(Geom g, LinePlan lp
   }
   if constexpr (P::NS > 0) {
     if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
-      p.finalize(acc);
+      finalize_copy(p, acc);
   }
 }
 
@@ -541,6 +552,31 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
   slab_store(S, lds, base, n);
 }
 
+// Last inverse pass of a row (radix R) into registers, then the consumer on
+// batches of kPairs output pairs.
+template <typename T, int EMAX, int R, class P>
+__device__ __forceinline__ void c2r_finish(const P& p, const Geom& g, int row, int tl, const LinePlan& lp,
+                                           const cplx<T>* lds, int rb, const cplx<T>* __restrict__ tw, double* acc) {
+  const int N2 = g.N2;
+  cplx<T> v[EMAX];
+  last_pass_regs<T, R, true, EMAX>(v, lds, rb, 1, N2, tl, lp.tpl, tw);
+  const int nbl = nbutterflies(N2, R, tl, lp.tpl);
+#pragma unroll
+  for (int e0 = 0; e0 < EMAX; e0 += kPairs) {
+    typename P::Rec rec[kPairs];
+#pragma unroll
+    for (int k = 0; k < kPairs; ++k) {
+      const int e = e0 + k;
+      if (e / R < nbl) rec[k] = p.fetch(g, row, 2 * slot_element(e, R, N2, tl, lp.tpl));
+    }
+#pragma unroll
+    for (int k = 0; k < kPairs; ++k) {
+      const int e = e0 + k;
+      if (e / R < nbl) p.take(g, row, 2 * slot_element(e, R, N2, tl, lp.tpl), rec[k], v[e].x, v[e].y, acc);
+    }
+  }
+}
+
 // Row pass: packed half spectrum -> real rows (inverse, unnormalised; the
 // round trip r2c -> c2r scales by Mx*My, which the kernels G/K absorb).
 template <typename T, int EMAX, class P>
@@ -587,32 +623,25 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_c2r(Geom g, LinePlan lp
     }
   }
   __syncthreads();
-  cplx<T> v[EMAX];
-  fft_line_x<T, true, EMAX, false, true>(v, lds, rb, 1, lp, tl, tw);
+  fft_line_to_last<T, true, EMAX>(lds, rb, 1, lp, tl, tw);
   double acc[P::NS > 0 ? P::NS : 1];
 #pragma unroll
   for (int i = 0; i < P::NS; ++i) acc[i] = red_identity(P::op(i));
-  {
-    const int Rl = lp.rlast;
-    const int nbl = nbutterflies(N2, Rl, tl, lp.tpl);
-#pragma unroll
-    for (int e0 = 0; e0 < EMAX; e0 += kPairs) {
-      typename P::Rec rec[kPairs];
-#pragma unroll
-      for (int k = 0; k < kPairs; ++k) {
-        const int e = e0 + k;
-        if (e / Rl < nbl) rec[k] = p.fetch(g, row, 2 * slot_element(e, Rl, N2, tl, lp.tpl));
-      }
-#pragma unroll
-      for (int k = 0; k < kPairs; ++k) {
-        const int e = e0 + k;
-        if (e / Rl < nbl) p.take(g, row, 2 * slot_element(e, Rl, N2, tl, lp.tpl), rec[k], v[e].x, v[e].y, acc);
-      }
-    }
+  // last inverse pass into registers and the consumer, specialised on the
+  // last radix so every register slot index is a compile-time constant
+  switch (lp.rlast) {
+    case 2: c2r_finish<T, EMAX, 2>(p, g, row, tl, lp, lds, rb, tw, acc); break;
+    case 3: c2r_finish<T, EMAX, 3>(p, g, row, tl, lp, lds, rb, tw, acc); break;
+    case 4: c2r_finish<T, EMAX, 4>(p, g, row, tl, lp, lds, rb, tw, acc); break;
+    case 5: c2r_finish<T, EMAX, 5>(p, g, row, tl, lp, lds, rb, tw, acc); break;
+    case 16:
+      if constexpr (EMAX >= 16) c2r_finish<T, EMAX, 16>(p, g, row, tl, lp, lds, rb, tw, acc);
+      break;
+    default: c2r_finish<T, EMAX, 8>(p, g, row, tl, lp, lds, rb, tw, acc); break;
   }
   if constexpr (P::NS > 0) {
     if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
-      p.finalize(acc);
+      finalize_copy(p, acc);
   }
 }
 
@@ -644,7 +673,7 @@ __global__ void __launch_bounds__(256) k_elem(Geom g, InstArr<T> ia, int ctr_slo
   }
   if constexpr (P::NS > 0) {
     if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
-      p.finalize(acc);
+      finalize_copy(p, acc);
   }
 }
 
