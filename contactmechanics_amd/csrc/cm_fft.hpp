@@ -175,49 +175,92 @@ struct LinePlan {
 // Twiddle table: tw[k] = exp(-2 pi i k / L), k in [0, L).
 // One Stockham pass (radix R, sub-transform length Ns before the pass).
 // Line element n lives at lds[base + n*stride].
-template <typename T, int R, bool INV, int EMAX>
+template <typename T, int R, bool INV, int EMAX, bool EARLY = true>
 __device__ __forceinline__ void stockham_pass(cplx<T>* lds, int base, int stride, int L, int Ns,
                                               int tl, int tpl, int nb, const cplx<T>* __restrict__ tw) {
   constexpr int NBMAX = EMAX / R > 0 ? EMAX / R : 1;
   cplx<T> v[NBMAX][R];
   const int LR = L / R;
-#pragma unroll
-  for (int b = 0; b < NBMAX; ++b) {
-    if (b < nb) {
-      const int j = tl + b * tpl;
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = lds[lpad<T>(base + (j + r * LR) * stride)];
-    }
-  }
-  __syncthreads();
   const int twstep = L / (Ns * R);
   // j = q * Ns + k; Ns is uniform, and a power of two for 2^n lengths
   const bool pow2 = (Ns & (Ns - 1)) == 0;
   const int nsh = __builtin_ctz(Ns);
+  auto split = [&](int j, int& q, int& k) {
+    if (pow2) {
+      q = j >> nsh;
+      k = j & (Ns - 1);
+    } else {
+      q = j / Ns;
+      k = j - q * Ns;
+    }
+  };
+  if constexpr (EARLY) {
+    // twiddles first: they are global (cached) loads independent of the LDS
+    // contents, so their latency overlaps the LDS reads and the barrier (at
+    // the price of holding them in registers across the barrier)
+    int qq[NBMAX], kk[NBMAX];
+    cplx<T> w[NBMAX][R - 1];
 #pragma unroll
-  for (int b = 0; b < NBMAX; ++b) {
-    if (b < nb) {
-      const int j = tl + b * tpl;
-      int q, k;
-      if (pow2) {
-        q = j >> nsh;
-        k = j & (Ns - 1);
-      } else {
-        q = j / Ns;
-        k = j - q * Ns;
-      }
-      if (Ns > 1) {
-        const int kt = k * twstep;
+    for (int b = 0; b < NBMAX; ++b) {
+      if (b < nb) {
+        split(tl + b * tpl, qq[b], kk[b]);
+        if (Ns > 1) {
+          const int kt = kk[b] * twstep;
 #pragma unroll
-        for (int r = 1; r < R; ++r) {
-          cplx<T> w = tw[r * kt];
-          v[b][r] = INV ? cmulc(v[b][r], w) : cmul(v[b][r], w);
+          for (int r = 1; r < R; ++r) w[b][r - 1] = tw[r * kt];
         }
       }
-      Dft<T, R, INV>::run(v[b]);
-      const int o = q * Ns * R + k;
+    }
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[lpad<T>(base + (o + r * Ns) * stride)] = v[b][r];
+    for (int b = 0; b < NBMAX; ++b) {
+      if (b < nb) {
+        const int j = tl + b * tpl;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b][r] = lds[lpad<T>(base + (j + r * LR) * stride)];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NBMAX; ++b) {
+      if (b < nb) {
+        if (Ns > 1) {
+#pragma unroll
+          for (int r = 1; r < R; ++r) v[b][r] = INV ? cmulc(v[b][r], w[b][r - 1]) : cmul(v[b][r], w[b][r - 1]);
+        }
+        Dft<T, R, INV>::run(v[b]);
+        const int o = qq[b] * Ns * R + kk[b];
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[lpad<T>(base + (o + r * Ns) * stride)] = v[b][r];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int b = 0; b < NBMAX; ++b) {
+      if (b < nb) {
+        const int j = tl + b * tpl;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b][r] = lds[lpad<T>(base + (j + r * LR) * stride)];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NBMAX; ++b) {
+      if (b < nb) {
+        int q, k;
+        split(tl + b * tpl, q, k);
+        if (Ns > 1) {
+          const int kt = k * twstep;
+#pragma unroll
+          for (int r = 1; r < R; ++r) {
+            const cplx<T> t = tw[r * kt];
+            v[b][r] = INV ? cmulc(v[b][r], t) : cmul(v[b][r], t);
+          }
+        }
+        Dft<T, R, INV>::run(v[b]);
+        const int o = q * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[lpad<T>(base + (o + r * Ns) * stride)] = v[b][r];
+      }
     }
   }
   __syncthreads();
@@ -233,31 +276,31 @@ __device__ __forceinline__ int nbutterflies(int L, int R, int tl, int tpl) {
 
 // One Stockham pass of radix R (R <= EMAX; radix 16 only exists in kernels
 // built for 16 or more register elements, so smaller builds stay lean).
-template <typename T, bool INV, int EMAX>
+template <typename T, bool INV, int EMAX, bool EARLY = true>
 __device__ __forceinline__ void pass_dispatch(int R, cplx<T>* lds, int base, int stride, int L, int Ns, int tl,
                                               int tpl, int nb, const cplx<T>* __restrict__ tw) {
   switch (R) {
-    case 2: stockham_pass<T, 2, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
-    case 3: stockham_pass<T, 3, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
-    case 4: stockham_pass<T, 4, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
-    case 5: stockham_pass<T, 5, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
+    case 2: stockham_pass<T, 2, INV, EMAX, EARLY>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
+    case 3: stockham_pass<T, 3, INV, EMAX, EARLY>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
+    case 4: stockham_pass<T, 4, INV, EMAX, EARLY>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
+    case 5: stockham_pass<T, 5, INV, EMAX, EARLY>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
     case 16:
-      if constexpr (EMAX >= 16) stockham_pass<T, 16, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw);
+      if constexpr (EMAX >= 16) stockham_pass<T, 16, INV, EMAX, EARLY>(lds, base, stride, L, Ns, tl, tpl, nb, tw);
       break;
-    default: stockham_pass<T, 8, INV, EMAX>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
+    default: stockham_pass<T, 8, INV, EMAX, EARLY>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
   }
 }
 
 // Full in-place LDS FFT of one line (all threads of the block must call it;
 // threads whose tl >= tpl must pass active=false and still hit the barriers).
-template <typename T, bool INV, int EMAX>
+template <typename T, bool INV, int EMAX, bool EARLY = true>
 __device__ void fft_line(cplx<T>* lds, int base, int stride, const LinePlan& p, int tl, bool active,
                          const cplx<T>* __restrict__ tw) {
   int Ns = 1;
   for (int s = 0; s < p.npass; ++s) {
     const int R = p.radix[s];
     const int nb = active ? nbutterflies(p.L, R, tl, p.tpl) : 0;
-    pass_dispatch<T, INV, EMAX>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw);
+    pass_dispatch<T, INV, EMAX, EARLY>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw);
     Ns *= R;
   }
 }
@@ -325,7 +368,7 @@ __device__ __forceinline__ void last_pass_regs(cplx<T>* v, const cplx<T>* lds, i
 // in LDS (natural order), and the output left in registers (OUT_REGS,
 // last-pass order) or in LDS (natural order).  With OUT_REGS the caller must
 // synchronise before writing LDS again.
-template <typename T, bool INV, int EMAX, bool IN_REGS, bool OUT_REGS>
+template <typename T, bool INV, int EMAX, bool IN_REGS, bool OUT_REGS, bool EARLY = true>
 __device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const LinePlan& p, int tl,
                            const cplx<T>* __restrict__ tw) {
   int s0 = 0, s1 = p.npass, Ns = 1;
@@ -348,7 +391,7 @@ __device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const
   for (int s = s0; s < s1; ++s) {
     const int R = p.radix[s];
     const int nb = nbutterflies(p.L, R, tl, p.tpl);
-    pass_dispatch<T, INV, EMAX>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw);
+    pass_dispatch<T, INV, EMAX, EARLY>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw);
     Ns *= R;
   }
   if (OUT_REGS) {
@@ -377,13 +420,13 @@ __device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const
 // All passes but the last, from LDS (natural order) to LDS; the caller runs
 // the last pass itself (last_pass_regs with its radix as a template argument,
 // so the register slots it fills are compile-time indices).
-template <typename T, bool INV, int EMAX>
+template <typename T, bool INV, int EMAX, bool EARLY = true>
 __device__ __forceinline__ void fft_line_to_last(cplx<T>* lds, int base, int stride, const LinePlan& p, int tl,
                                  const cplx<T>* __restrict__ tw) {
   int Ns = 1;
   for (int s = 0; s + 1 < p.npass; ++s) {
     const int R = p.radix[s];
-    pass_dispatch<T, INV, EMAX>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nbutterflies(p.L, R, tl, p.tpl), tw);
+    pass_dispatch<T, INV, EMAX, EARLY>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nbutterflies(p.L, R, tl, p.tpl), tw);
     Ns *= R;
   }
 }

@@ -412,6 +412,11 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_r2c(Geom g, LinePlan lp
   }
 }
 
+// Column kernels load twiddles early (before each pass's barrier) only in
+// FP32: in FP64 the extra registers would cost the 8-element column kernel
+// its fourth wave per SIMD.
+template <typename T> constexpr bool kColEarlyTw = sizeof(T) == 4;
+
 __device__ void tau_finalize(Scal* s, double sum_rt);
 
 // Reduction slots of the spectral tau pass (one sum).
@@ -456,7 +461,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
 #pragma unroll
       for (int e = 0; e < EMAX; ++e)
         if (e / R0 < nb0) v[e] = Sg[base + (size_t)slot_element(e, R0, Mx, tl, lp.tpl) * Cb + c];
-      fft_line_x<T, false, EMAX, true, true>(v, lds, c, Cb, lp, tl, tw);
+      fft_line_x<T, false, EMAX, true, true, kColEarlyTw<T>>(v, lds, c, Cb, lp, tl, tw);
       const auto Gk = (const __attribute__((address_space(1))) T*)(G + (size_t)(cb * Cb + c) * Mx);
 #pragma unroll
       for (int e = 0; e < EMAX; ++e)
@@ -467,7 +472,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
     } else {
       slab_load(lds, S, base, n);
       __syncthreads();
-      fft_line<T, false, EMAX>(lds, c, Cb, lp, tl, true, tw);
+      fft_line<T, false, EMAX, kColEarlyTw<T>>(lds, c, Cb, lp, tl, true, tw);
       if (c != 0) {
         const T* Gk = G + (size_t)c * Mx;
         for (int x = tl; x < Mx; x += lp.tpl) {
@@ -504,7 +509,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
 #pragma unroll
     for (int e = 0; e < EMAX; ++e)
       if (e / R0 < nb0) v[e] = Sg[base + (size_t)slot_element(e, R0, Mx, tl, lp.tpl) * Cb + c];
-    fft_line_x<T, false, EMAX, true, true>(v, lds, c, Cb, lp, tl, tw);
+    fft_line_x<T, false, EMAX, true, true, kColEarlyTw<T>>(v, lds, c, Cb, lp, tl, tw);
     __syncthreads();
     const auto Gk = (const __attribute__((address_space(1))) T*)(G + (size_t)(cb * Cb + c) * Mx);
 #pragma unroll
@@ -512,7 +517,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
       if (e / Rl < nbl) v[e] = v[e] * Gk[slot_element(e, Rl, Mx, tl, lp.tpl)];
     // rv: the plan with its radices reversed, so the inverse transform's first
     // pass takes the forward transform's last-pass register slots as they are
-    fft_line_x<T, true, EMAX, true, true>(v, lds, c, Cb, rv, tl, tw);
+    fft_line_x<T, true, EMAX, true, true, kColEarlyTw<T>>(v, lds, c, Cb, rv, tl, tw);
     const GViewW<T> Sw(S);
 #pragma unroll
     for (int e = 0; e < EMAX; ++e)
@@ -522,7 +527,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
   slab_load(lds, S, base, n);
   __syncthreads();
   const int c = threadIdx.x % Cb, tl = threadIdx.x / Cb;
-  if (MODE != 2) fft_line<T, false, EMAX>(lds, c, Cb, lp, tl, true, tw);
+  if (MODE != 2) fft_line<T, false, EMAX, kColEarlyTw<T>>(lds, c, Cb, lp, tl, true, tw);
   if (MODE == 0) {
     const int ky = cb * Cb + c;
     if (ky != 0) {
@@ -548,7 +553,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
     }
     __syncthreads();
   }
-  if (MODE != 1) fft_line<T, true, EMAX>(lds, c, Cb, lp, tl, true, tw);
+  if (MODE != 1) fft_line<T, true, EMAX, kColEarlyTw<T>>(lds, c, Cb, lp, tl, true, tw);
   slab_store(S, lds, base, n);
 }
 
