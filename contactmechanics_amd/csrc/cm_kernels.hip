@@ -628,21 +628,40 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_c2r(Geom g, LinePlan lp
     }
   }
   __syncthreads();
-  fft_line_to_last<T, true, EMAX>(lds, rb, 1, lp, tl, tw);
   double acc[P::NS > 0 ? P::NS : 1];
 #pragma unroll
   for (int i = 0; i < P::NS; ++i) acc[i] = red_identity(P::op(i));
-  // last inverse pass into registers and the consumer, specialised on the
-  // last radix so every register slot index is a compile-time constant
-  switch (lp.rlast) {
-    case 2: c2r_finish<T, EMAX, 2>(p, g, row, tl, lp, lds, rb, tw, acc); break;
-    case 3: c2r_finish<T, EMAX, 3>(p, g, row, tl, lp, lds, rb, tw, acc); break;
-    case 4: c2r_finish<T, EMAX, 4>(p, g, row, tl, lp, lds, rb, tw, acc); break;
-    case 5: c2r_finish<T, EMAX, 5>(p, g, row, tl, lp, lds, rb, tw, acc); break;
-    case 16:
-      if constexpr (EMAX >= 16) c2r_finish<T, EMAX, 16>(p, g, row, tl, lp, lds, rb, tw, acc);
-      break;
-    default: c2r_finish<T, EMAX, 8>(p, g, row, tl, lp, lds, rb, tw, acc); break;
+  if constexpr (EMAX == 8) {
+    // last inverse pass into registers and the consumer, specialised on the
+    // last radix so every register slot index is a compile-time constant
+    fft_line_to_last<T, true, EMAX>(lds, rb, 1, lp, tl, tw);
+    switch (lp.rlast) {
+      case 2: c2r_finish<T, EMAX, 2>(p, g, row, tl, lp, lds, rb, tw, acc); break;
+      case 3: c2r_finish<T, EMAX, 3>(p, g, row, tl, lp, lds, rb, tw, acc); break;
+      case 4: c2r_finish<T, EMAX, 4>(p, g, row, tl, lp, lds, rb, tw, acc); break;
+      case 5: c2r_finish<T, EMAX, 5>(p, g, row, tl, lp, lds, rb, tw, acc); break;
+      default: c2r_finish<T, EMAX, 8>(p, g, row, tl, lp, lds, rb, tw, acc); break;
+    }
+  } else {
+    // long lines (16/32-element builds): one generic consumer loop
+    cplx<T> v[EMAX];
+    fft_line_x<T, true, EMAX, false, true>(v, lds, rb, 1, lp, tl, tw);
+    const int Rl = lp.rlast;
+    const int nbl = nbutterflies(N2, Rl, tl, lp.tpl);
+#pragma unroll
+    for (int e0 = 0; e0 < EMAX; e0 += kPairs) {
+      typename P::Rec rec[kPairs];
+#pragma unroll
+      for (int k = 0; k < kPairs; ++k) {
+        const int e = e0 + k;
+        if (e / Rl < nbl) rec[k] = p.fetch(g, row, 2 * slot_element(e, Rl, N2, tl, lp.tpl));
+      }
+#pragma unroll
+      for (int k = 0; k < kPairs; ++k) {
+        const int e = e0 + k;
+        if (e / Rl < nbl) p.take(g, row, 2 * slot_element(e, Rl, N2, tl, lp.tpl), rec[k], v[e].x, v[e].y, acc);
+      }
+    }
   }
   if constexpr (P::NS > 0) {
     if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
@@ -1739,9 +1758,11 @@ int plan_build(cm_plan* P) {
   const size_t M = (size_t)P->Mx * P->My;
   // element-wise kernels: blocks stride over rows; few enough blocks that the
   // last block's pass over the partials stays short
-  // element-wise passes: about one row of work per block, up to 2048 blocks
-  // (eight waves per CU on 256 CUs) so enough loads are in flight
-  P->nblk_elem = (int)std::max<size_t>(1, std::min<size_t>(std::min<size_t>(2048, (size_t)P->Mx), (M + 1023) / 1024));
+  // element-wise passes: up to 1024 blocks (four per CU); blocks loop over
+  // rows, so waves drift out of step and loads of one overlap the arithmetic
+  // and reductions of another (measured at 2048^2: 1024 blocks beat 512 and
+  // 2048 by 4-7 us per pass)
+  P->nblk_elem = (int)std::max<size_t>(1, std::min<size_t>(std::min<size_t>(1024, (size_t)P->Mx), (M + 1023) / 1024));
   return 0;
 }
 
