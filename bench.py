@@ -116,22 +116,27 @@ def rel_errors(p, ref):
     return dict(l2=l2, linf=linf, area_rel=abs(a - b) / max(b, 1))
 
 
-def cpu_baseline(cfg, iters, substrate_gpu):
-    """FP64 NumPy oracle on the same workload: time `iters` iterations."""
+def cpu_baseline(cfg, iters, substrate_gpu, workers=None):
+    """FP64 NumPy oracle on the same workload: time `iters` iterations (FFTs on
+    `workers` threads via scipy.fft when given, else numpy on one thread)."""
     import numpy as np
+    from oracle import halfspace as ohs
     from oracle.ccg import constrained_conjugate_gradients as ccg_cpu
-    from oracle.halfspace import PeriodicHalfSpace
     n = cfg["n"]
-    hs = PeriodicHalfSpace((n, n), 1.0, (1.0, 1.0))
+    hs = ohs.PeriodicHalfSpace((n, n), 1.0, (1.0, 1.0))
     kw = dict(external_force=cfg["external_force"], pentol=0.0, forcetol=0.0, thermotol=0.0)
-    # set-up + 1 iteration, then set-up + 1 + iters iterations: the
-    # difference times exactly `iters` iterations
-    t0 = time.perf_counter()
-    ccg_cpu(hs, cfg["heights"], maxiter=1, **kw)
-    t_one = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    ref = ccg_cpu(hs, cfg["heights"], maxiter=iters + 1, **kw)
-    t_run = time.perf_counter() - t0 - t_one
+    ohs.set_fft_workers(workers)
+    try:
+        # set-up + 1 iteration, then set-up + 1 + iters iterations: the
+        # difference times exactly `iters` iterations
+        t0 = time.perf_counter()
+        ccg_cpu(hs, cfg["heights"], maxiter=1, **kw)
+        t_one = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ref = ccg_cpu(hs, cfg["heights"], maxiter=iters + 1, **kw)
+        t_run = time.perf_counter() - t0 - t_one
+    finally:
+        ohs.set_fft_workers(None)
     # same iterations on the GPU (FP64) for a same-seed error figure
     from contactmechanics_amd import constrained_conjugate_gradients as ccg_gpu
     g = ccg_gpu(substrate_gpu, cfg["heights"], maxiter=iters + 1, **kw)
@@ -319,6 +324,24 @@ def main():
             fp32_vs_fp64_pressure_err=err)
         print(f"# solve fp64 nit={r64.nit} {t64:.3f}s; fp32 nit={r32.nit} {t32:.3f}s; err {err}",
               file=sys.stderr, flush=True)
+        # precision of the paths themselves: all three to tight tolerances
+        # (gtol 1e-10), errors against the FP64 solve
+        from contactmechanics_amd import PeriodicFFTElasticHalfSpace as PS
+        tight = dict(cfg, **workloads.tolerances(1e-10, n, n))
+        tight["thermotol"] = 1e-10
+        submx = PS((n, n), 1.0, (1.0, 1.0), precision="mixed")
+        t64t = full_solve(sub64, cfg["heights"], tight)
+        tmx = full_solve(submx, cfg["heights"], tight)
+        t32t = full_solve(sub32, cfg["heights"], tight, maxiter=3000)
+        extra["solve_tight"] = dict(
+            gtol=1e-10,
+            fp64=dict(nit=int(t64t[0].nit), seconds=round(t64t[1], 4), success=bool(t64t[0].success)),
+            mixed=dict(nit=int(tmx[0].nit), nit_fp32=int(getattr(tmx[0], "nit_fp32", -1)),
+                       nit_fp64=int(getattr(tmx[0], "nit_fp64", -1)), seconds=round(tmx[1], 4),
+                       success=bool(tmx[0].success), vs_fp64=rel_errors(tmx[0].jac, t64t[0].jac)),
+            fp32=dict(nit=int(t32t[0].nit), seconds=round(t32t[1], 4), success=bool(t32t[0].success),
+                      vs_fp64=rel_errors(t32t[0].jac, t64t[0].jac)))
+        print(f"# tight: {extra['solve_tight']}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -330,6 +353,15 @@ def main():
                "gpu_fp64_vs_oracle_linf_after_same_iterations": cb["gpu_vs_oracle_linf"],
                "host": platform.processor() or platform.machine()}
         print(f"# cpu: {cb}", file=sys.stderr, flush=True)
+        # multi-core variant: the same oracle with its FFTs on every core this
+        # process may use (capped at 16, the host share of one GPU)
+        nw = min(16, len(os.sched_getaffinity(0)))
+        if nw > 1:
+            cbt = cpu_baseline(cfg, args.cpu_iters, sub64, workers=nw)
+            cpu["threaded"] = {"value": M * cbt["iterations"] / cbt["seconds"], "cores": nw,
+                               "sample": f"same {cbt['iterations']} iterations, scipy.fft with {nw} workers, "
+                                         f"element-wise NumPy on one thread, {cbt['seconds']:.2f}s"}
+            print(f"# cpu threaded ({nw}): {cbt}", file=sys.stderr, flush=True)
 
     out = {
         "metric": "CCG grid-pt-iters/s (FP64 path, rel-err vs FP64 oracle <= 1e-6)",

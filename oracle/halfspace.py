@@ -44,10 +44,23 @@ def _tuple2(v, dim):
     return tuple(v[min(i, len(v) - 1)] for i in range(dim))
 
 
+# FP64 transforms use numpy's pocketfft (one thread); set_fft_workers(n)
+# switches them to scipy.fft with n threads (used to time a multi-core CPU
+# baseline; same transforms up to round-off).
+_FFT_WORKERS = None
+
+
+def set_fft_workers(n):
+    global _FFT_WORKERS
+    _FFT_WORKERS = n
+
+
 def _rfft2_axis0(a, dtype):
     """Unnormalised r2c along axis 0 -> shape (nx//2+1, ny)."""
     if dtype == np.float32:
         return scipy.fft.rfft2(np.asarray(a, dtype=np.float32), axes=(1, 0))
+    if _FFT_WORKERS:
+        return scipy.fft.rfft2(a, axes=(1, 0), workers=_FFT_WORKERS)
     return np.fft.rfft2(a, axes=(1, 0))
 
 
@@ -56,6 +69,8 @@ def _irfft2_axis0(a, shape, dtype):
     if dtype == np.float32:
         return scipy.fft.irfft2(np.asarray(a, dtype=np.complex64), s=shape[::-1],
                                 axes=(1, 0))
+    if _FFT_WORKERS:
+        return scipy.fft.irfft2(a, s=shape[::-1], axes=(1, 0), workers=_FFT_WORKERS)
     return np.fft.irfft2(a, s=shape[::-1], axes=(1, 0))
 
 
