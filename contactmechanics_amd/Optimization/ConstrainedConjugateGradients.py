@@ -254,9 +254,12 @@ def _mixed_precision(substrate, topography, pentol=None, forcetol=1e-5, thermoto
     rms = _rms_height(heights)
     scale_f = abs(external_force) / (nx * ny) if external_force is not None else 0.0
     twin = substrate.fp32_twin()
+    # FP32 stage: tolerances at the level FP32 fields can resolve (relative
+    # 1e-5 of the height and pixel-force scales; tighter ones stall at large
+    # grids) and at most half the iteration budget; the FP64 stage finishes
     r32 = constrained_conjugate_gradients(
-        twin, topography, external_force=external_force, pentol=max(pentol, 1e-6 * rms),
-        forcetol=max(forcetol, 1e-6 * scale_f), thermotol=max(thermotol, 1e-6), maxiter=maxiter,
+        twin, topography, external_force=external_force, pentol=max(pentol, 2e-5 * rms),
+        forcetol=max(forcetol, 1e-5 * scale_f), thermotol=max(thermotol, 1e-5), maxiter=max(1, maxiter // 2),
         return_device=True, **kw)
     f0 = torch.zeros(tuple(substrate.nb_subdomain_grid_pts), dtype=torch.float64, device=substrate._device_str())
     f0[:nx, :ny] = -r32.jac.to(torch.float64)
