@@ -1,0 +1,93 @@
+"""Run one emulated case (a subprocess with AddressSanitizer preloaded) and
+print a JSON line: solves or operator evaluations through the C ABI of the
+host-built kernels, compared with the oracle."""
+
+import ctypes
+import itertools
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from contactmechanics_amd import _native as N  # noqa: E402  (ctypes structs only)
+from oracle.ccg import constrained_conjugate_gradients as ccg  # noqa: E402
+from oracle.halfspace import FreeHalfSpace, PeriodicHalfSpace  # noqa: E402
+
+lib = ctypes.CDLL(os.path.join(HERE, "_build", "libcmemu.so"))
+lib.cm_last_error.restype = ctypes.c_char_p
+ctx = ctypes.c_void_p()
+assert lib.cm_ctx_create(0, ctypes.c_size_t(0), ctypes.byref(ctx)) == 0
+
+
+def plan(nx, ny, bc, prec, sx=1.0, sy=1.0, young=1.0):
+    d = N.SubstrateDesc(nx=nx, ny=ny, bc=bc, precision=prec, sx=sx, sy=sy, young=young, poisson=0.0,
+                        q0_mode=0, stiffness_q0=0.0, thickness=float("nan"))
+    p = ctypes.c_void_p()
+    rc = lib.cm_plan_create(ctx, ctypes.byref(d), ctypes.byref(p))
+    assert rc == 0, lib.cm_last_error(ctx)
+    mx, my = ctypes.c_int(), ctypes.c_int()
+    lib.cm_plan_grid(p, ctypes.byref(mx), ctypes.byref(my))
+    return p, mx.value, my.value
+
+
+def heights(nx, ny, seed):
+    from contactmechanics_amd import workloads
+    return workloads.self_affine(max(nx, ny), 0.8, 0.1, seed)[:nx, :ny].copy()
+
+
+def solve(case):
+    nx, ny, bc, prec = case["nx"], case["ny"], case["bc"], case["prec"]
+    p, mx, my = plan(nx, ny, bc, prec)
+    h = heights(nx, ny, case["seed"])
+    kw = dict(external_force=case["force"], pentol=case["pentol"], forcetol=case["forcetol"], maxiter=600)
+    if case.get("hardness"):
+        kw["hardness"] = case["hardness"]
+    dt = np.float64 if prec == 0 else np.float32
+    u = np.zeros((mx, my), dt)
+    f = np.zeros((mx, my), dt)
+    cm = np.zeros((mx, my), np.uint8)
+    prm = N.CcgParams(load_control=1, external_force=kw["external_force"], offset=float("nan"),
+                      has_hardness=int("hardness" in kw), hardness=kw.get("hardness", 0.0),
+                      pentol=kw["pentol"], forcetol=kw["forcetol"], thermotol=1e-6, mixfac=0.1, mixdecfac=0.95,
+                      minmixsteps=10, maxiter=kw["maxiter"], init_mode=0, iters_per_sync=0, use_graph=0)
+    st = N.CcgStats()
+    hc = np.ascontiguousarray(h)
+    vp = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
+    rc = lib.cm_ccg_solve(p, ctypes.byref(prm), vp(hc), None, vp(u), vp(f), vp(cm), ctypes.byref(st),
+                          N.PROGRESS_FN(), None, None)
+    cls = FreeHalfSpace if bc else PeriodicHalfSpace
+    r = ccg(cls((nx, ny), 1.0, (1.0, 1.0)), h, **kw)
+    jac = -f[:nx, :ny].astype(np.float64)
+    return dict(rc=rc, nit=st.nit, nit_oracle=int(r.nit), success_oracle=bool(r.success),
+                rel=float(np.max(np.abs(jac - r.jac)) / np.max(np.abs(r.jac))))
+
+
+def operators(case):
+    nx, ny, bc = case["nx"], case["ny"], case["bc"]
+    p, mx, my = plan(nx, ny, bc, 0, 1.0, 2.0, 1.7)
+    c = (FreeHalfSpace if bc else PeriodicHalfSpace)((nx, ny), 1.7, (1.0, 2.0))
+    f = np.random.default_rng(nx + ny).standard_normal((mx, my))
+    ud, fd = c.evaluate_disp(f), c.evaluate_force(f)
+    worst, n = 0.0, 0
+    for sh in [None] + list(itertools.product((1, 2, 4, 8), (1, 2, 4, 8), (8, 16, 32), (8, 16, 32))):
+        if sh is not None and lib.cm_plan_set_shape(p, *sh) != 0:
+            continue
+        out = np.zeros((mx, my))
+        assert lib.cm_evaluate_disp(p, ctypes.c_void_p(f.ctypes.data), ctypes.c_void_p(out.ctypes.data), None) == 0
+        e1 = np.max(np.abs(out - ud)) / np.max(np.abs(ud))
+        assert lib.cm_evaluate_force(p, ctypes.c_void_p(f.ctypes.data), ctypes.c_void_p(out.ctypes.data), None) == 0
+        e2 = np.max(np.abs(out - fd)) / np.max(np.abs(fd))
+        worst = max(worst, e1, e2 / 10)
+        n += 1
+    return dict(shapes=n, worst=worst)
+
+
+if __name__ == "__main__":
+    case = json.loads(sys.argv[1])
+    out = solve(case) if case["kind"] == "solve" else operators(case)
+    print(json.dumps(out), flush=True)
