@@ -80,3 +80,26 @@ def test_torch_device_path_and_errors(gpu):
     # 7 and 14 are not 2^a 3^b 5^c: the plan refuses loudly (no CPU fallback)
     with pytest.raises(PeriodicFFTElasticHalfSpace.Error):
         PeriodicFFTElasticHalfSpace((7, 14), 1.0, (1.0, 1.0)).evaluate_disp(np.zeros((7, 14)))
+
+
+@pytest.mark.parametrize("nx,ny", [(4000, 32), (32, 4000), (2430, 48)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_long_mixed_radix_lines(gpu, nx, ny, prec):
+    # free grids of 8000-point lines (8000 = 16*4*5^3) take the 16/32-element
+    # kernels with blocks of more than 256 threads; 2430 = 2*3^5*5 periodic
+    rng = np.random.default_rng(nx + 3 * ny)
+    if nx == 2430:
+        g = PeriodicFFTElasticHalfSpace((nx, ny), 1.3, (2.0, 3.0), precision=prec)
+        c = PeriodicHalfSpace((nx, ny), 1.3, (2.0, 3.0))
+        f = rng.standard_normal((nx, ny))
+        tol_d = tol_f = 1e-12 if prec == "fp64" else 3e-6
+    else:
+        g = FreeFFTElasticHalfSpace((nx, ny), 1.3, (2.0, 3.0), precision=prec)
+        c = FreeHalfSpace((nx, ny), 1.3, (2.0, 3.0))
+        f = rng.standard_normal(g.nb_domain_grid_pts)
+        # evaluate_force divides by the Love-kernel spectrum, whose smallest
+        # (high-frequency) values carry the FFT round-off of an 8000-point
+        # line relative to its largest: 2e-8 observed, against 1e-12 for disp
+        tol_d, tol_f = (1e-9, 1e-7) if prec == "fp64" else (3e-6, 3e-6)
+    assert rel(g.evaluate_disp(f), c.evaluate_disp(f)) < tol_d
+    assert rel(g.evaluate_force(f), c.evaluate_force(f)) < tol_f
