@@ -67,7 +67,8 @@ struct Scal {
 
 template <typename T>
 struct Inst {
-  T* u;            // displacements (M grid)
+  T* u;            // displacements (M grid); during the iterations of a solve
+                   // u - h on the computational points (see ElemPhaseA)
   T* f;            // forces (M grid)
   T* t;            // conjugate direction (M grid)
   uint8_t* cm;     // bit0 contact set c, bit1 gap<0 at start of iteration
@@ -241,6 +242,23 @@ __device__ __forceinline__ HPair ldh(const Geom& g, const T* h, const uint8_t* s
   } else {
     r.h0 = r.c0 ? (double)h[q] : 0.0;
     r.h1 = r.c1 ? (double)h[q + 1] : 0.0;
+  }
+  return r;
+}
+
+// Validity of the pair (row, col), (row, col+1) alone, for the passes that
+// work on the stored gap u - h and need no heights.
+struct CPair {
+  bool c0, c1;
+};
+__device__ __forceinline__ CPair ldc(const Geom& g, const uint8_t* surf, int row, int col) {
+  CPair r;
+  r.c0 = row < g.nx && col < g.ny;
+  r.c1 = row < g.nx && col + 1 < g.ny;
+  if (surf) {
+    const size_t q = (size_t)row * g.ny + col;
+    r.c0 = r.c0 && surf[q];
+    r.c1 = r.c1 && surf[q + 1];
   }
   return r;
 }
@@ -849,17 +867,16 @@ __device__ void phase_a_logic(Scal* s, const double* a) {
 }
 
 // Per-point part of the start of an iteration: contact set from the current
-// forces (:239-249) and the gap sums shifted by `shift` (the offset of the
-// previous iteration, which keeps sum (g - offset)^2 free of cancellation);
-// returns c.
-__device__ __forceinline__ bool phase_a_point(const Scal* s, double shift, double fv, double uv, double hv,
-                                              bool comp, double* acc) {
+// forces (:239-249) and the sums of the gap g0 = u - h shifted by `shift` (the
+// offset of the previous iteration, which keeps sum (g - offset)^2 free of
+// cancellation); returns c.
+__device__ __forceinline__ bool phase_a_point(const Scal* s, double shift, double fv, double g0, bool comp,
+                                              double* acc) {
   const bool c0 = fv < 0.0;
   const bool c = c0 && (!s->has_H || fv > -s->H);
   acc[0] += c0 ? 1.0 : 0.0;
   acc[1] += c ? 1.0 : 0.0;
   if (comp) {
-    const double g0 = uv - hv;
     const double d = g0 - shift;
     if (c) acc[2] += d;                  // sum over the contact set only (:269)
     acc[3] += (c ? 1.0 : 0.0) * d * d;   // c * g * g: a NaN gap poisons G as in :273
@@ -875,7 +892,13 @@ __device__ __forceinline__ bool phase_a_point(const Scal* s, double shift, doubl
 constexpr uint8_t kCur = 1, kGneg = 2, kNext = 4;
 
 // Phase A of the first iteration (later iterations fuse it into ConsDisp):
-// contact set and gap statistics (:238-273, :312-318 brackets).
+// contact set and gap statistics (:238-273, :312-318 brackets).  It also
+// replaces u by the gap u - h (:265) on the computational points: every later
+// pass of the iteration uses only the gap, so the heights are read once per
+// iteration (by ConsDisp, which stores u_new - h); ElemFinish restores u.
+// The gap is the one the oracle forms, fl(u - h), rounded to the field type.
+// It runs also for an instance that is already done (maxiter < 1), so that
+// ElemFinish always finds the gap.
 template <typename T>
 struct ElemPhaseA {
   static constexpr int NS = 6;  // A_contact, A_cg, Sd, Sdd, gmax, gmin
@@ -886,18 +909,24 @@ struct ElemPhaseA {
   };
   const Inst<T>& in;
   __device__ ElemPhaseA(const Inst<T>& i) : in(i) {}
-  __device__ bool skip() const { return in.sc->done != 0; }
+  __device__ bool skip() const { return false; }
   __device__ Rec fetch(const Geom& g, int row, int col) const {
     const size_t p = (size_t)row * g.My + col;
     return Rec{ld2(in.f + p), ld2(in.u + p), ldh(g, in.h, in.surf, row, col)};
   }
   __device__ void emit(const Geom& g, int row, int col, const Rec& r, double* acc) const {
     const Scal* s = in.sc;
-    const bool c0 = phase_a_point(s, s->shift, (double)r.f.a, (double)r.u.a, r.h.h0, r.h.c0, acc);
-    const bool c1 = phase_a_point(s, s->shift, (double)r.f.b, (double)r.u.b, r.h.h1, r.h.c1, acc);
-    stm(in.cm + (size_t)row * g.My + col, c0 ? kNext : 0, c1 ? kNext : 0);
+    const T w0 = r.h.c0 ? (T)((double)r.u.a - r.h.h0) : r.u.a;
+    const T w1 = r.h.c1 ? (T)((double)r.u.b - r.h.h1) : r.u.b;
+    const bool c0 = phase_a_point(s, s->shift, (double)r.f.a, (double)w0, r.h.c0, acc);
+    const bool c1 = phase_a_point(s, s->shift, (double)r.f.b, (double)w1, r.h.c1, acc);
+    const size_t p = (size_t)row * g.My + col;
+    stm(in.cm + p, c0 ? kNext : 0, c1 ? kNext : 0);
+    if (r.h.c0 || r.h.c1) st2(in.u + p, w0, w1);
   }
-  __device__ void finalize(double* a) const { phase_a_logic(in.sc, a); }
+  __device__ void finalize(double* a) const {
+    if (!in.sc->done) phase_a_logic(in.sc, a);
+  }
 };
 
 // scipy.optimize.bisect (xtol=2e-12, rtol=4*eps, maxiter=100) of
@@ -917,7 +946,7 @@ __global__ void __launch_bounds__(1024) k_bisect(Geom g, InstArr<T> ia) {
     for (size_t q = threadIdx.x; q < Mt; q += blockDim.x) {
       const int i = (int)(q / g.ny), j = (int)(q - (size_t)i * g.ny);
       if (in.surf && !in.surf[q]) continue;
-      const double g0 = (double)in.u[(size_t)i * g.My + j] - (double)in.h[q];
+      const double g0 = (double)in.u[(size_t)i * g.My + j];  // u holds the gap u - h
       v += (g0 - x < 0.0) ? 1.0 : 0.0;
     }
     int ops[1] = {RSUM};
@@ -972,20 +1001,21 @@ struct ProdDir {
   static constexpr int NS = 1;
   __device__ static constexpr int op(int) { return RSUM; }
   struct Rec {
-    P2<T> u, t;
+    P2<T> w, t;
     P2<uint8_t> cm;
-    HPair h;
+    CPair cp;
   };
   const Inst<T>& in;
   __device__ ProdDir(const Inst<T>& i) : in(i) {}
   __device__ bool skip() const { return in.sc->done || in.sc->mode == MODE_MIX; }
   __device__ Rec fetch(const Geom& g, int row, int col) const {
     const size_t p = (size_t)row * g.My + col;
-    return Rec{ld2(in.u + p), ld2(in.t + p), ldm(in.cm + p), ldh(g, in.h, in.surf, row, col)};
+    return Rec{ld2(in.u + p), ld2(in.t + p), ldm(in.cm + p), ldc(g, in.surf, row, col)};
   }
-  __device__ T one(const Scal* s, bool comp, bool c, double uv, double hv, T told, double* acc) const {
+  // wv: the stored gap u - h
+  __device__ T one(const Scal* s, bool comp, bool c, double wv, T told, double* acc) const {
     if (!comp) return told;  // t is only ever set on comp (zero elsewhere)
-    const double gap = (uv - hv) - s->offset;
+    const double gap = wv - s->offset;
     double tn = 0.0;
     if (c) tn = s->use_beta ? gap + s->beta * (double)told : gap;
     const T tt = (T)tn;
@@ -994,8 +1024,8 @@ struct ProdDir {
   }
   __device__ void make(const Geom& g, int row, int col, const Rec& r, T& a, T& b, double* acc) const {
     const Scal* s = in.sc;
-    a = one(s, r.h.c0, r.cm.a & kNext, (double)r.u.a, r.h.h0, r.t.a, acc);
-    b = one(s, r.h.c1, r.cm.b & kNext, (double)r.u.b, r.h.h1, r.t.b, acc);
+    a = one(s, r.cp.c0, r.cm.a & kNext, (double)r.w.a, r.t.a, acc);
+    b = one(s, r.cp.c1, r.cm.b & kNext, (double)r.w.b, r.t.b, acc);
     st2(in.t + (size_t)row * g.My + col, a, b);
   }
   __device__ void finalize(double* a) const { in.sc->Scgt = a[0]; }
@@ -1029,21 +1059,22 @@ struct ElemPhaseB {
     return (i == 1 || i == 2) ? RMAX : (i == 4 ? RMIN : RSUM);
   }
   struct Rec {
-    P2<T> f, t, u;
+    P2<T> f, t, w;
     P2<uint8_t> cm;
-    HPair h;
+    CPair cp;
   };
   const Inst<T>& in;
   __device__ ElemPhaseB(const Inst<T>& i) : in(i) {}
   __device__ bool skip() const { return in.sc->done != 0; }
   __device__ Rec fetch(const Geom& g, int row, int col) const {
     const size_t p = (size_t)row * g.My + col;
-    return Rec{ld2(in.f + p), ld2(in.t + p), ld2(in.u + p), ldm(in.cm + p), ldh(g, in.h, in.surf, row, col)};
+    return Rec{ld2(in.f + p), ld2(in.t + p), ld2(in.u + p), ldm(in.cm + p), ldc(g, in.surf, row, col)};
   }
-  __device__ uint8_t one(const Scal* s, bool comp, uint8_t cmv, double fv, double tv, double uv, double hv,
+  // wv: the stored gap u - h
+  __device__ uint8_t one(const Scal* s, bool comp, uint8_t cmv, double fv, double tv, double wv,
                          double* acc) const {
     double gap = 0.0;
-    if (comp) gap = (uv - hv) - s->offset;
+    if (comp) gap = wv - s->offset;
     bool c = (cmv & kNext) != 0;
     if (s->mode == MODE_MIX && comp) c = gap < 0.0;  // c[comp] = g < 0 (:322)
     const double f1 = force_update(s, fv, tv, c);
@@ -1067,8 +1098,8 @@ struct ElemPhaseB {
   }
   __device__ void emit(const Geom& g, int row, int col, const Rec& r, double* acc) const {
     const Scal* s = in.sc;
-    const uint8_t c0 = one(s, r.h.c0, r.cm.a, (double)r.f.a, (double)r.t.a, (double)r.u.a, r.h.h0, acc);
-    const uint8_t c1 = one(s, r.h.c1, r.cm.b, (double)r.f.b, (double)r.t.b, (double)r.u.b, r.h.h1, acc);
+    const uint8_t c0 = one(s, r.cp.c0, r.cm.a, (double)r.f.a, (double)r.t.a, (double)r.w.a, acc);
+    const uint8_t c1 = one(s, r.cp.c1, r.cm.b, (double)r.f.b, (double)r.t.b, (double)r.w.b, acc);
     if (s->mode == MODE_MIX) stm(in.cm + (size_t)row * g.My + col, c0, c1);
   }
   __device__ void finalize(double* a) const {
@@ -1111,21 +1142,22 @@ struct ProdForce {
   static constexpr int NS = 1;  // sum f over comp
   __device__ static constexpr int op(int) { return RSUM; }
   struct Rec {
-    P2<T> f, t, u;
+    P2<T> f, t, w;
     P2<uint8_t> cm;
-    HPair h;
+    CPair cp;
   };
   const Inst<T>& in;
   __device__ ProdForce(const Inst<T>& i) : in(i) {}
   __device__ bool skip() const { return in.sc->done != 0; }
   __device__ Rec fetch(const Geom& g, int row, int col) const {
     const size_t p = (size_t)row * g.My + col;
-    return Rec{ld2(in.f + p), ld2(in.t + p), ld2(in.u + p), ldm(in.cm + p), ldh(g, in.h, in.surf, row, col)};
+    return Rec{ld2(in.f + p), ld2(in.t + p), ld2(in.u + p), ldm(in.cm + p), ldc(g, in.surf, row, col)};
   }
-  __device__ T one(const Scal* s, bool comp, uint8_t& cmv, double fv, double tv, double uv, double hv,
+  // wv: the stored gap u - h
+  __device__ T one(const Scal* s, bool comp, uint8_t& cmv, double fv, double tv, double wv,
                    double* acc) const {
     double gap = 0.0;
-    if (comp) gap = (uv - hv) - s->offset;
+    if (comp) gap = wv - s->offset;
     const bool c = (cmv & kNext) != 0;
     const double f1 = force_update(s, fv, tv, c);
     const bool tensile = f1 >= 0.0;
@@ -1150,11 +1182,11 @@ struct ProdForce {
   __device__ void make(const Geom& g, int row, int col, const Rec& r, T& a, T& b, double* acc) const {
     const Scal* s = in.sc;
     uint8_t m0 = r.cm.a, m1 = r.cm.b;
-    a = one(s, r.h.c0, m0, (double)r.f.a, (double)r.t.a, (double)r.u.a, r.h.h0, acc);
-    b = one(s, r.h.c1, m1, (double)r.f.b, (double)r.t.b, (double)r.u.b, r.h.h1, acc);
+    a = one(s, r.cp.c0, m0, (double)r.f.a, (double)r.t.a, (double)r.w.a, acc);
+    b = one(s, r.cp.c1, m1, (double)r.f.b, (double)r.t.b, (double)r.w.b, acc);
     const size_t p = (size_t)row * g.My + col;
     st2(in.f + p, a, b);
-    if (r.h.c0 || r.h.c1) stm(in.cm + p, m0, m1);
+    if (r.cp.c0 || r.cp.c1) stm(in.cm + p, m0, m1);
   }
   __device__ void finalize(double* a) const { in.sc->total_force = -a[0]; }
 };
@@ -1163,6 +1195,10 @@ struct ProdForce {
 // convergence tests (:391-434), NaN check (:491-492), maxiter (:494-509);
 // fused with the start of the next iteration (contact set from the final
 // forces, shifted gap sums of u_new), so no separate phase-A pass is needed.
+// The displacement buffer keeps the gap u - h on the computational points:
+// the old gap is read, u_new - h stored; maxdu and max_pen are formed from
+// the gaps (u_new - u_old = gap_new - gap_old, h + offset - u_new =
+// offset - gap_new), which is the only place the heights are read.
 template <typename T>
 struct ConsDisp {
   static constexpr int NS = 8;  // maxdu, max_pen | A_contact, A_cg, Sd, Sdd, gmax, gmin (next iteration)
@@ -1181,23 +1217,27 @@ struct ConsDisp {
     const size_t p = (size_t)row * g.My + col;
     return Rec{ld2(in.u + p), ld2(in.f + p), ldm(in.cm + p), ldh(g, in.h, in.surf, row, col)};
   }
-  __device__ uint8_t one(const Scal* s, bool comp, uint8_t cmv, double uo, double fv, double hv, T v,
+  // wo: stored value (gap on comp, displacement elsewhere); v: u_new;
+  // w: the new stored value
+  __device__ uint8_t one(const Scal* s, bool comp, uint8_t cmv, double wo, double fv, double hv, T v, T& w,
                          double* acc) const {
-    const double un = (double)v;
-    acc[0] = red_apply(RMAX, acc[0], fabs(un - uo));
+    w = comp ? (T)((double)v - hv) : v;
+    const double wn = (double)w;
+    acc[0] = red_apply(RMAX, acc[0], fabs(wn - wo));
     if (comp) {
-      const double pen = (cmv & kCur) ? (hv + s->offset - un) : 0.0;
+      const double pen = (cmv & kCur) ? (s->offset - wn) : 0.0;
       acc[1] = red_apply(RMAX, acc[1], pen);
     }
-    const bool cn = phase_a_point(s, s->offset, fv, un, hv, comp, acc + 2);  // next shift = this offset
+    const bool cn = phase_a_point(s, s->offset, fv, wn, comp, acc + 2);  // next shift = this offset
     return (uint8_t)((cmv & (kCur | kGneg)) | (cn ? kNext : 0));
   }
   __device__ void take(const Geom& g, int row, int col, const Rec& r, T a, T b, double* acc) const {
     const Scal* s = in.sc;
-    const uint8_t m0 = one(s, r.h.c0, r.cm.a, (double)r.u.a, (double)r.f.a, r.h.h0, a, acc);
-    const uint8_t m1 = one(s, r.h.c1, r.cm.b, (double)r.u.b, (double)r.f.b, r.h.h1, b, acc);
+    T w0, w1;
+    const uint8_t m0 = one(s, r.h.c0, r.cm.a, (double)r.u.a, (double)r.f.a, r.h.h0, a, w0, acc);
+    const uint8_t m1 = one(s, r.h.c1, r.cm.b, (double)r.u.b, (double)r.f.b, r.h.h1, b, w1, acc);
     const size_t p = (size_t)row * g.My + col;
-    st2(in.u + p, a, b);
+    st2(in.u + p, w0, w1);
     stm(in.cm + p, m0, m1);
   }
   __device__ void finalize(double* a) const {
@@ -1247,24 +1287,30 @@ struct ConsDisp {
   }
 };
 
-// Elastic energy -sum(f u)/2 over the topography block (:471, :505-506).
+// End of a solve: the displacements u = gap + h back on the computational
+// points (undoing ElemPhaseA's substitution) and the elastic energy
+// -sum(f u)/2 over the topography block (:471, :505-506).
 template <typename T>
-struct ElemEnergy {
+struct ElemFinish {
   static constexpr int NS = 1;
   __device__ static constexpr int op(int) { return RSUM; }
   struct Rec {
-    P2<T> f, u;
+    P2<T> f, w;
+    HPair h;
   };
   const Inst<T>& in;
-  __device__ ElemEnergy(const Inst<T>& i) : in(i) {}
+  __device__ ElemFinish(const Inst<T>& i) : in(i) {}
   __device__ bool skip() const { return false; }
   __device__ Rec fetch(const Geom& g, int row, int col) const {
     const size_t p = (size_t)row * g.My + col;
-    return Rec{ld2(in.f + p), ld2(in.u + p)};
+    return Rec{ld2(in.f + p), ld2(in.u + p), ldh(g, in.h, in.surf, row, col)};
   }
   __device__ void emit(const Geom& g, int row, int col, const Rec& r, double* acc) const {
-    if (row < g.nx && col < g.ny) acc[0] += (double)r.f.a * (double)r.u.a;
-    if (row < g.nx && col + 1 < g.ny) acc[0] += (double)r.f.b * (double)r.u.b;
+    const T u0 = r.h.c0 ? (T)((double)r.w.a + r.h.h0) : r.w.a;
+    const T u1 = r.h.c1 ? (T)((double)r.w.b + r.h.h1) : r.w.b;
+    if (r.h.c0 || r.h.c1) st2(in.u + (size_t)row * g.My + col, u0, u1);
+    if (row < g.nx && col < g.ny) acc[0] += (double)r.f.a * (double)u0;
+    if (row < g.nx && col + 1 < g.ny) acc[0] += (double)r.f.b * (double)u1;
   }
   __device__ void finalize(double* a) const { in.sc->fun = -a[0] / 2; }
 };
@@ -1932,7 +1978,7 @@ struct cm_session {
   int use_graph = 0;
   long long issued = 0;
   int maxiter = 0;
-  bool finished = false;
+  bool finished = false;  // ElemFinish has restored the displacements
   Scal* host = nullptr;  // pinned copy of the B scalar records
   void* hin = nullptr;   // pinned instance tables (main, set-up)
   void* hin0 = nullptr;
@@ -2138,7 +2184,8 @@ int session_finish(cm_session* S) {
   const FftExec& ex = S->ex;
   InstArr<T> ia{};
   ia.arr = (const Inst<T>*)((char*)S->ws + S->L.inst);
-  launch_elem<T, ElemEnergy<T>>(ex, S->P->nblk_elem, ia, S->B, 0, S->st);
+  S->finished = true;
+  launch_elem<T, ElemFinish<T>>(ex, S->P->nblk_elem, ia, S->B, 0, S->st);
   return 0;
 }
 
@@ -2444,6 +2491,7 @@ int cm_ccg_begin(cm_plan* plan, int B, const cm_ccg_params* params, const double
 
 int cm_ccg_step(cm_session* s, int n_iters) {
   if (!s) return fail(nullptr, CM_E_ARG, "null session");
+  if (s->finished) return fail(s->P->ctx, CM_E_ARG, "session already finished");
   return s->P->prec == CM_FP64 ? session_step<double>(s, n_iters) : session_step<float>(s, n_iters);
 }
 
@@ -2454,7 +2502,8 @@ int cm_ccg_poll(cm_session* s, cm_ccg_stats* stats, int* all_done) {
 
 int cm_ccg_finish(cm_session* s, cm_ccg_stats* stats) {
   if (!s || !stats) return fail(nullptr, CM_E_ARG, "null argument");
-  s->P->prec == CM_FP64 ? session_finish<double>(s) : session_finish<float>(s);
+  // a second call only reports: the displacements are restored once
+  if (!s->finished) s->P->prec == CM_FP64 ? session_finish<double>(s) : session_finish<float>(s);
   return session_poll(s, stats, nullptr);
 }
 
@@ -2465,6 +2514,7 @@ int cm_ccg_end(cm_session* s) {
 
 int cm_ccg_profile(cm_session* s, int n_iters, float* kernel_us) {
   if (!s || !kernel_us || n_iters < 1) return fail(nullptr, CM_E_ARG, "bad argument");
+  if (s->finished) return fail(s->P->ctx, CM_E_ARG, "session already finished");
   return s->P->prec == CM_FP64 ? session_profile<double>(s, n_iters, kernel_us)
                                : session_profile<float>(s, n_iters, kernel_us);
 }

@@ -167,7 +167,10 @@ int cm_ccg_solve_batch(cm_plan* plan, int B, const cm_ccg_params* params,
  * (ConstrainedConjugateGradients.py:179-219) asynchronously on `stream`;
  * cm_ccg_step enqueues n iterations (finished instances turn into no-ops);
  * cm_ccg_poll synchronises and reads the statistics; cm_ccg_finish computes
- * the elastic energy; cm_ccg_end frees the workspace.  Sessions on different
+ * the elastic energy; cm_ccg_end frees the workspace.  Between begin and
+ * finish the u buffers hold u - h on the topography points (the heights are
+ * read once per iteration that way); cm_ccg_finish restores u, after which
+ * cm_ccg_step and cm_ccg_profile return CM_E_ARG.  Sessions on different
  * streams run concurrently.  No counterpart in the reference, which only has
  * the blocking call replaced by cm_ccg_solve. */
 size_t cm_ccg_session_bytes(cm_plan* plan, int B);
