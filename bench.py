@@ -32,18 +32,20 @@ def kernel_bytes(name, M, Mt, w):
     """Bytes each fused kernel must read + write once per launch (FP width w;
     heights 8 bytes on the FP64 path and 4 on the FP32 path, contact masks one
     byte).  The start of an iteration (contact set, gap sums) is fused into
-    the displacement pass, so "phase_a" moves no bytes per iteration."""
+    the displacement pass, so "phase_a" moves no bytes per iteration.  The
+    solver keeps the gap u - h in the displacement buffer, so only the
+    displacement pass reads the heights."""
     hw = 8 if w == 8 else 4
     return {
         "phase_a": 0,
         "bisect": 0,
-        "dir_r2c": M * (4 * w + 1) + hw * Mt,
+        "dir_r2c": M * (4 * w + 1),
         # forward column FFT of the direction and sum_k G|T|^2 (reads S and G,
         # writes nothing): the step length needs no inverse pass
         "col_conv1": int(1.5 * w * M),
         "tau_c2r": 0,
-        "phase_b": M * (3 * w + 1) + hw * Mt,
-        "force_r2c": M * (5 * w + 2) + hw * Mt,
+        "phase_b": M * (3 * w + 1),
+        "force_r2c": M * (5 * w + 2),
         "col_conv2": int(2.5 * w * M),
         "disp_c2r": M * (4 * w + 2) + hw * Mt,
     }[name]
