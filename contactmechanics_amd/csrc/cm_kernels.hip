@@ -1006,8 +1006,9 @@ struct ProdDir {
     CPair cp;
   };
   const Inst<T>& in;
-  __device__ ProdDir(const Inst<T>& i) : in(i) {}
-  __device__ bool skip() const { return in.sc->done || in.sc->mode == MODE_MIX; }
+  const Scal sv;  // scalar record as the kernel began: only the last block's finalize writes it
+  __device__ ProdDir(const Inst<T>& i) : in(i), sv(*i.sc) {}
+  __device__ bool skip() const { return sv.done || sv.mode == MODE_MIX; }
   __device__ Rec fetch(const Geom& g, int row, int col) const {
     const size_t p = (size_t)row * g.My + col;
     return Rec{ld2(in.u + p), ld2(in.t + p), ldm(in.cm + p), ldc(g, in.surf, row, col)};
@@ -1023,7 +1024,7 @@ struct ProdDir {
     return tt;
   }
   __device__ void make(const Geom& g, int row, int col, const Rec& r, T& a, T& b, double* acc) const {
-    const Scal* s = in.sc;
+    const Scal* s = &sv;
     a = one(s, r.cp.c0, r.cm.a & kNext, (double)r.w.a, r.t.a, acc);
     b = one(s, r.cp.c1, r.cm.b & kNext, (double)r.w.b, r.t.b, acc);
     st2(in.t + (size_t)row * g.My + col, a, b);
@@ -1064,8 +1065,9 @@ struct ElemPhaseB {
     CPair cp;
   };
   const Inst<T>& in;
-  __device__ ElemPhaseB(const Inst<T>& i) : in(i) {}
-  __device__ bool skip() const { return in.sc->done != 0; }
+  const Scal sv;  // scalar record as the kernel began: only the last block's finalize writes it
+  __device__ ElemPhaseB(const Inst<T>& i) : in(i), sv(*i.sc) {}
+  __device__ bool skip() const { return sv.done != 0; }
   __device__ Rec fetch(const Geom& g, int row, int col) const {
     const size_t p = (size_t)row * g.My + col;
     return Rec{ld2(in.f + p), ld2(in.t + p), ld2(in.u + p), ldm(in.cm + p), ldc(g, in.surf, row, col)};
@@ -1097,7 +1099,7 @@ struct ElemPhaseB {
     return (uint8_t)((cmv & ~kNext) | (c ? kNext : 0));
   }
   __device__ void emit(const Geom& g, int row, int col, const Rec& r, double* acc) const {
-    const Scal* s = in.sc;
+    const Scal* s = &sv;
     const uint8_t c0 = one(s, r.cp.c0, r.cm.a, (double)r.f.a, (double)r.t.a, (double)r.w.a, acc);
     const uint8_t c1 = one(s, r.cp.c1, r.cm.b, (double)r.f.b, (double)r.t.b, (double)r.w.b, acc);
     if (s->mode == MODE_MIX) stm(in.cm + (size_t)row * g.My + col, c0, c1);
