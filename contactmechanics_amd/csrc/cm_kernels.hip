@@ -184,7 +184,8 @@ __device__ __forceinline__ bool reduce_blocks(double* acc, double* part, unsigne
 // Per-point functors work on pairs of horizontally adjacent points (col even):
 //   Rec fetch(g, row, col)                 loads only;
 //   make(g, row, col, rec, a, b, acc)      producer: outputs + FFT input pair;
-//   take(g, row, col, rec, a, b, acc)      consumer of an FFT output pair;
+//   take(g, row, col, rec, a, b, acc, cx)  consumer of an FFT output pair
+//                                          (cx = ctx(): scalars read once per thread);
 //   emit(g, row, col, rec, acc)            element-wise kernel body.
 // Kernels fetch a batch of kPairs pairs per thread before computing any, so
 // the loads are in flight together instead of waiting behind stores.
@@ -582,6 +583,7 @@ __device__ __forceinline__ void c2r_finish(const P& p, const Geom& g, int row, i
                                            const cplx<T>* lds, int rb, const cplx<T>* __restrict__ tw, double* acc) {
   const int N2 = g.N2;
   cplx<T> v[EMAX];
+  const typename P::Ctx cx = p.ctx();
   last_pass_regs<T, R, true, EMAX>(v, lds, rb, 1, N2, tl, lp.tpl, tw);
   const int nbl = nbutterflies(N2, R, tl, lp.tpl);
 #pragma unroll
@@ -595,7 +597,7 @@ __device__ __forceinline__ void c2r_finish(const P& p, const Geom& g, int row, i
 #pragma unroll
     for (int k = 0; k < kPairs; ++k) {
       const int e = e0 + k;
-      if (e / R < nbl) p.take(g, row, 2 * slot_element(e, R, N2, tl, lp.tpl), rec[k], v[e].x, v[e].y, acc);
+      if (e / R < nbl) p.take(g, row, 2 * slot_element(e, R, N2, tl, lp.tpl), rec[k], v[e].x, v[e].y, acc, cx);
     }
   }
 }
@@ -663,6 +665,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_c2r(Geom g, LinePlan lp
   } else {
     // long lines (16/32-element builds): one generic consumer loop
     cplx<T> v[EMAX];
+    const typename P::Ctx cx = p.ctx();
     fft_line_x<T, true, EMAX, false, true>(v, lds, rb, 1, lp, tl, tw);
     const int Rl = lp.rlast;
     const int nbl = nbutterflies(N2, Rl, tl, lp.tpl);
@@ -677,7 +680,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_c2r(Geom g, LinePlan lp
 #pragma unroll
       for (int k = 0; k < kPairs; ++k) {
         const int e = e0 + k;
-        if (e / Rl < nbl) p.take(g, row, 2 * slot_element(e, Rl, N2, tl, lp.tpl), rec[k], v[e].x, v[e].y, acc);
+        if (e / Rl < nbl) p.take(g, row, 2 * slot_element(e, Rl, N2, tl, lp.tpl), rec[k], v[e].x, v[e].y, acc, cx);
       }
     }
   }
@@ -749,7 +752,9 @@ struct ConsCopy {  // output row to dst
   __device__ ConsCopy(const Inst<T>& i) : in(i) {}
   __device__ bool skip() const { return false; }
   __device__ Rec fetch(const Geom&, int, int) const { return Rec{}; }
-  __device__ void take(const Geom& g, int row, int col, const Rec&, T a, T b, double*) const {
+  struct Ctx {};
+  __device__ Ctx ctx() const { return Ctx{}; }
+  __device__ void take(const Geom& g, int row, int col, const Rec&, T a, T b, double*, const Ctx&) const {
     st2(in.dst + (size_t)row * g.My + col, a, b);
   }
   __device__ void finalize(double*) const {}
@@ -810,7 +815,9 @@ struct ConsInitForce {
   __device__ ConsInitForce(const Inst<T>& i) : in(i) {}
   __device__ bool skip() const { return false; }
   __device__ Rec fetch(const Geom& g, int row, int col) const { return Rec{ldh(g, in.h, in.surf, row, col)}; }
-  __device__ void take(const Geom& g, int row, int col, const Rec& r, T a, T b, double*) const {
+  struct Ctx {};
+  __device__ Ctx ctx() const { return Ctx{}; }
+  __device__ void take(const Geom& g, int row, int col, const Rec& r, T a, T b, double*, const Ctx&) const {
     st2(in.f + (size_t)row * g.My + col, r.h.c0 ? a : T(0), r.h.c1 ? b : T(0));
   }
   __device__ void finalize(double*) const {}
@@ -870,10 +877,10 @@ __device__ void phase_a_logic(Scal* s, const double* a) {
 // forces (:239-249) and the sums of the gap g0 = u - h shifted by `shift` (the
 // offset of the previous iteration, which keeps sum (g - offset)^2 free of
 // cancellation); returns c.
-__device__ __forceinline__ bool phase_a_point(const Scal* s, double shift, double fv, double g0, bool comp,
+__device__ __forceinline__ bool phase_a_point(bool has_H, double H, double shift, double fv, double g0, bool comp,
                                               double* acc) {
   const bool c0 = fv < 0.0;
-  const bool c = c0 && (!s->has_H || fv > -s->H);
+  const bool c = c0 && (!has_H || fv > -H);
   acc[0] += c0 ? 1.0 : 0.0;
   acc[1] += c ? 1.0 : 0.0;
   if (comp) {
@@ -918,8 +925,8 @@ struct ElemPhaseA {
     const Scal* s = in.sc;
     const T w0 = r.h.c0 ? (T)((double)r.u.a - r.h.h0) : r.u.a;
     const T w1 = r.h.c1 ? (T)((double)r.u.b - r.h.h1) : r.u.b;
-    const bool c0 = phase_a_point(s, s->shift, (double)r.f.a, (double)w0, r.h.c0, acc);
-    const bool c1 = phase_a_point(s, s->shift, (double)r.f.b, (double)w1, r.h.c1, acc);
+    const bool c0 = phase_a_point(s->has_H, s->H, s->shift, (double)r.f.a, (double)w0, r.h.c0, acc);
+    const bool c1 = phase_a_point(s->has_H, s->H, s->shift, (double)r.f.b, (double)w1, r.h.c1, acc);
     const size_t p = (size_t)row * g.My + col;
     stm(in.cm + p, c0 ? kNext : 0, c1 ? kNext : 0);
     if (r.h.c0 || r.h.c1) st2(in.u + p, w0, w1);
@@ -1149,8 +1156,9 @@ struct ProdForce {
     CPair cp;
   };
   const Inst<T>& in;
-  __device__ ProdForce(const Inst<T>& i) : in(i) {}
-  __device__ bool skip() const { return in.sc->done != 0; }
+  const Scal sv;  // scalar record as the kernel began: only the last block's finalize writes it
+  __device__ ProdForce(const Inst<T>& i) : in(i), sv(*i.sc) {}
+  __device__ bool skip() const { return sv.done != 0; }
   __device__ Rec fetch(const Geom& g, int row, int col) const {
     const size_t p = (size_t)row * g.My + col;
     return Rec{ld2(in.f + p), ld2(in.t + p), ld2(in.u + p), ldm(in.cm + p), ldc(g, in.surf, row, col)};
@@ -1182,7 +1190,7 @@ struct ProdForce {
     return ft;
   }
   __device__ void make(const Geom& g, int row, int col, const Rec& r, T& a, T& b, double* acc) const {
-    const Scal* s = in.sc;
+    const Scal* s = &sv;
     uint8_t m0 = r.cm.a, m1 = r.cm.b;
     a = one(s, r.cp.c0, m0, (double)r.f.a, (double)r.t.a, (double)r.w.a, acc);
     b = one(s, r.cp.c1, m1, (double)r.f.b, (double)r.t.b, (double)r.w.b, acc);
@@ -1219,25 +1227,33 @@ struct ConsDisp {
     const size_t p = (size_t)row * g.My + col;
     return Rec{ld2(in.u + p), ld2(in.f + p), ldm(in.cm + p), ldh(g, in.h, in.surf, row, col)};
   }
+  // The scalars the consumer needs, read once per thread just before the
+  // last transform pass (per point pair they would be dependent loads after
+  // every store; across the whole transform they would cost registers).
+  struct Ctx {
+    double offset, H;
+    bool has_H;
+  };
+  __device__ Ctx ctx() const { return Ctx{in.sc->offset, in.sc->H, in.sc->has_H != 0}; }
   // wo: stored value (gap on comp, displacement elsewhere); v: u_new;
   // w: the new stored value
-  __device__ uint8_t one(const Scal* s, bool comp, uint8_t cmv, double wo, double fv, double hv, T v, T& w,
+  __device__ uint8_t one(const Ctx& cx, bool comp, uint8_t cmv, double wo, double fv, double hv, T v, T& w,
                          double* acc) const {
     w = comp ? (T)((double)v - hv) : v;
     const double wn = (double)w;
     acc[0] = red_apply(RMAX, acc[0], fabs(wn - wo));
     if (comp) {
-      const double pen = (cmv & kCur) ? (s->offset - wn) : 0.0;
+      const double pen = (cmv & kCur) ? (cx.offset - wn) : 0.0;
       acc[1] = red_apply(RMAX, acc[1], pen);
     }
-    const bool cn = phase_a_point(s, s->offset, fv, wn, comp, acc + 2);  // next shift = this offset
+    // next shift = this offset
+    const bool cn = phase_a_point(cx.has_H, cx.H, cx.offset, fv, wn, comp, acc + 2);
     return (uint8_t)((cmv & (kCur | kGneg)) | (cn ? kNext : 0));
   }
-  __device__ void take(const Geom& g, int row, int col, const Rec& r, T a, T b, double* acc) const {
-    const Scal* s = in.sc;
+  __device__ void take(const Geom& g, int row, int col, const Rec& r, T a, T b, double* acc, const Ctx& cx) const {
     T w0, w1;
-    const uint8_t m0 = one(s, r.h.c0, r.cm.a, (double)r.u.a, (double)r.f.a, r.h.h0, a, w0, acc);
-    const uint8_t m1 = one(s, r.h.c1, r.cm.b, (double)r.u.b, (double)r.f.b, r.h.h1, b, w1, acc);
+    const uint8_t m0 = one(cx, r.h.c0, r.cm.a, (double)r.u.a, (double)r.f.a, r.h.h0, a, w0, acc);
+    const uint8_t m1 = one(cx, r.h.c1, r.cm.b, (double)r.u.b, (double)r.f.b, r.h.h1, b, w1, acc);
     const size_t p = (size_t)row * g.My + col;
     st2(in.u + p, w0, w1);
     stm(in.cm + p, m0, m1);
