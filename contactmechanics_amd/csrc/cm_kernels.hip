@@ -577,7 +577,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
 }
 
 // Last inverse pass of a row (radix R) into registers, then the consumer on
-// batches of kPairs output pairs.
+// batches of P::kBatch output pairs.
 template <typename T, int EMAX, int R, class P>
 __device__ __forceinline__ void c2r_finish(const P& p, const Geom& g, int row, int tl, const LinePlan& lp,
                                            const cplx<T>* lds, int rb, const cplx<T>* __restrict__ tw, double* acc) {
@@ -587,15 +587,15 @@ __device__ __forceinline__ void c2r_finish(const P& p, const Geom& g, int row, i
   last_pass_regs<T, R, true, EMAX>(v, lds, rb, 1, N2, tl, lp.tpl, tw);
   const int nbl = nbutterflies(N2, R, tl, lp.tpl);
 #pragma unroll
-  for (int e0 = 0; e0 < EMAX; e0 += kPairs) {
-    typename P::Rec rec[kPairs];
+  for (int e0 = 0; e0 < EMAX; e0 += P::kBatch) {
+    typename P::Rec rec[P::kBatch];
 #pragma unroll
-    for (int k = 0; k < kPairs; ++k) {
+    for (int k = 0; k < P::kBatch; ++k) {
       const int e = e0 + k;
       if (e / R < nbl) rec[k] = p.fetch(g, row, 2 * slot_element(e, R, N2, tl, lp.tpl));
     }
 #pragma unroll
-    for (int k = 0; k < kPairs; ++k) {
+    for (int k = 0; k < P::kBatch; ++k) {
       const int e = e0 + k;
       if (e / R < nbl) p.take(g, row, 2 * slot_element(e, R, N2, tl, lp.tpl), rec[k], v[e].x, v[e].y, acc, cx);
     }
@@ -670,15 +670,15 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_c2r(Geom g, LinePlan lp
     const int Rl = lp.rlast;
     const int nbl = nbutterflies(N2, Rl, tl, lp.tpl);
 #pragma unroll
-    for (int e0 = 0; e0 < EMAX; e0 += kPairs) {
-      typename P::Rec rec[kPairs];
+    for (int e0 = 0; e0 < EMAX; e0 += P::kBatch) {
+      typename P::Rec rec[P::kBatch];
 #pragma unroll
-      for (int k = 0; k < kPairs; ++k) {
+      for (int k = 0; k < P::kBatch; ++k) {
         const int e = e0 + k;
         if (e / Rl < nbl) rec[k] = p.fetch(g, row, 2 * slot_element(e, Rl, N2, tl, lp.tpl));
       }
 #pragma unroll
-      for (int k = 0; k < kPairs; ++k) {
+      for (int k = 0; k < P::kBatch; ++k) {
         const int e = e0 + k;
         if (e / Rl < nbl) p.take(g, row, 2 * slot_element(e, Rl, N2, tl, lp.tpl), rec[k], v[e].x, v[e].y, acc, cx);
       }
@@ -746,6 +746,7 @@ struct ProdCopy {  // input row from src
 template <typename T>
 struct ConsCopy {  // output row to dst
   static constexpr int NS = 0;
+  static constexpr int kBatch = kPairs;
   __device__ static constexpr int op(int) { return RSUM; }
   struct Rec {};
   const Inst<T>& in;
@@ -807,6 +808,7 @@ struct ElemInit {
 template <typename T>
 struct ConsInitForce {
   static constexpr int NS = 0;
+  static constexpr int kBatch = kPairs;
   __device__ static constexpr int op(int) { return RSUM; }
   struct Rec {
     HPair h;
@@ -1012,7 +1014,7 @@ struct ProdDir {
     P2<uint8_t> cm;
     CPair cp;
   };
-  const Inst<T>& in;
+  const Inst<T> in;  // by value: the field pointers stay in registers (stores cannot alias them)
   const Scal sv;  // scalar record as the kernel began: only the last block's finalize writes it
   __device__ ProdDir(const Inst<T>& i) : in(i), sv(*i.sc) {}
   __device__ bool skip() const { return sv.done || sv.mode == MODE_MIX; }
@@ -1071,7 +1073,7 @@ struct ElemPhaseB {
     P2<uint8_t> cm;
     CPair cp;
   };
-  const Inst<T>& in;
+  const Inst<T> in;  // by value: the field pointers stay in registers (stores cannot alias them)
   const Scal sv;  // scalar record as the kernel began: only the last block's finalize writes it
   __device__ ElemPhaseB(const Inst<T>& i) : in(i), sv(*i.sc) {}
   __device__ bool skip() const { return sv.done != 0; }
@@ -1155,7 +1157,7 @@ struct ProdForce {
     P2<uint8_t> cm;
     CPair cp;
   };
-  const Inst<T>& in;
+  const Inst<T> in;  // by value: the field pointers stay in registers (stores cannot alias them)
   const Scal sv;  // scalar record as the kernel began: only the last block's finalize writes it
   __device__ ProdForce(const Inst<T>& i) : in(i), sv(*i.sc) {}
   __device__ bool skip() const { return sv.done != 0; }
@@ -1212,6 +1214,9 @@ struct ProdForce {
 template <typename T>
 struct ConsDisp {
   static constexpr int NS = 8;  // maxdu, max_pen | A_contact, A_cg, Sd, Sdd, gmax, gmin (next iteration)
+  // two output pairs per batch: the record by value and the transform's
+  // registers leave no room for four under 128 VGPRs (four waves per SIMD)
+  static constexpr int kBatch = 2;
   __device__ static constexpr int op(int i) {
     return i < 2 ? RMAX : (i == 6 ? RMAX : (i == 7 ? RMIN : RSUM));
   }
@@ -1220,7 +1225,7 @@ struct ConsDisp {
     P2<uint8_t> cm;
     HPair h;
   };
-  const Inst<T>& in;
+  const Inst<T> in;  // by value: the field pointers stay in registers (stores cannot alias them)
   __device__ ConsDisp(const Inst<T>& i) : in(i) {}
   __device__ bool skip() const { return in.sc->done != 0; }
   __device__ Rec fetch(const Geom& g, int row, int col) const {
