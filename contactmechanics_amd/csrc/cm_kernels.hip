@@ -226,29 +226,10 @@ struct HPair {
   double h0, h1;
   bool c0, c1;  // in the computational region
 };
-template <typename T>
-__device__ __forceinline__ HPair ldh(const Geom& g, const T* h, const uint8_t* surf, int row, int col) {
-  HPair r;
-  r.c0 = row < g.nx && col < g.ny;
-  r.c1 = row < g.nx && col + 1 < g.ny;
-  const size_t q = (size_t)row * g.ny + col;
-  if (surf) {
-    r.c0 = r.c0 && surf[q];
-    r.c1 = r.c1 && surf[q + 1];
-  }
-  if (r.c0 && r.c1 && (g.ny & 1) == 0) {  // both points valid and the pair aligned: one vector load
-    const P2<T> v = ld2(h + q);
-    r.h0 = (double)v.a;
-    r.h1 = (double)v.b;
-  } else {
-    r.h0 = r.c0 ? (double)h[q] : 0.0;
-    r.h1 = r.c1 ? (double)h[q + 1] : 0.0;
-  }
-  return r;
-}
-
-// Validity of the pair (row, col), (row, col+1) alone, for the passes that
-// work on the stored gap u - h and need no heights.
+// Validity of the pair (row, col), (row, col+1): in the topography block and
+// not masked out.  Branch-free (masked grids read the mask at a clamped,
+// always valid index) so that a batch of pair loads is not split by waits at
+// control-flow joins.
 struct CPair {
   bool c0, c1;
 };
@@ -257,10 +238,28 @@ __device__ __forceinline__ CPair ldc(const Geom& g, const uint8_t* surf, int row
   r.c0 = row < g.nx && col < g.ny;
   r.c1 = row < g.nx && col + 1 < g.ny;
   if (surf) {
+    const auto m = (const __attribute__((address_space(1))) uint8_t*)surf;
     const size_t q = (size_t)row * g.ny + col;
-    r.c0 = r.c0 && surf[q];
-    r.c1 = r.c1 && surf[q + 1];
+    const uint8_t m0 = m[r.c0 ? q : 0], m1 = m[r.c1 ? q + 1 : 0];
+    r.c0 = r.c0 && m0;
+    r.c1 = r.c1 && m1;
   }
+  return r;
+}
+
+// Heights and validity of the pair (row, col), (row, col+1); heights read at
+// clamped indices (no divergent branch) and zero off the valid points.
+template <typename T>
+__device__ __forceinline__ HPair ldh(const Geom& g, const T* h, const uint8_t* surf, int row, int col) {
+  const CPair c = ldc(g, surf, row, col);
+  const auto hg = (const __attribute__((address_space(1))) T*)h;
+  const size_t q = (size_t)row * g.ny + col;
+  const double h0 = (double)hg[c.c0 ? q : 0], h1 = (double)hg[c.c1 ? q + 1 : 0];
+  HPair r;
+  r.c0 = c.c0;
+  r.c1 = c.c1;
+  r.h0 = c.c0 ? h0 : 0.0;
+  r.h1 = c.c1 ? h1 : 0.0;
   return r;
 }
 
