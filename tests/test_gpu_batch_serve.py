@@ -102,3 +102,36 @@ def test_load_sweep_pressures_and_plastic(gpu):
                                                 maxiter=2000, warm_start=True)
     assert conv2.all()
     assert np.all(area2 >= area - 1e-12)  # a hardness cap can only spread the contact
+
+
+@pytest.mark.parametrize("bc", ["periodic", "free"])
+def test_session_restores_displacements(gpu, bc):
+    """During a session the displacement buffer holds the gap u - h; finish
+    restores u (oracle's result.x), also for a session that never iterates,
+    and stepping a finished session is refused."""
+    from contactmechanics_amd import FreeFFTElasticHalfSpace, PeriodicFFTElasticHalfSpace
+    from contactmechanics_amd.session import CcgSession, make_params
+    n = 32
+    p = problem(n, bc, 21)
+    gcls, ocls = ((PeriodicFFTElasticHalfSpace, PeriodicHalfSpace) if bc == "periodic"
+                  else (FreeFFTElasticHalfSpace, FreeHalfSpace))
+    sub = gcls((n, n), 1.0, (1.0, 1.0), precision="fp64")
+    s = CcgSession(sub, [p["heights"]], [make_params(external_force=p["external_force"], pentol=p["pentol"],
+                                                     forcetol=p["forcetol"], maxiter=5000)])
+    st = s.run()[0]
+    u = s.u[0].cpu().numpy()
+    ref = ccg_cpu(ocls((n, n), 1.0, (1.0, 1.0)), p["heights"], external_force=p["external_force"],
+                  pentol=p["pentol"], forcetol=p["forcetol"], maxiter=5000)
+    assert st["status"] == 0 and st["nit"] == ref.nit
+    assert np.max(np.abs(u - ref.x)) / np.max(np.abs(ref.x)) < 1e-10
+    with pytest.raises(Exception):
+        s.step(1)
+    s.close()
+    # no iteration at all: u comes back as the clamped initial displacement
+    s0 = CcgSession(sub, [p["heights"]], [make_params(offset=0.0, pentol=1e-8, forcetol=1e-5, maxiter=0)])
+    s0.finish()
+    u0 = s0.u[0].cpu().numpy()
+    expect = np.zeros_like(u0)
+    expect[:n, :n] = np.maximum(0.0, p["heights"])
+    assert np.array_equal(u0, expect)
+    s0.close()
