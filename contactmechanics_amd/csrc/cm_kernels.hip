@@ -227,9 +227,8 @@ struct HPair {
   bool c0, c1;  // in the computational region
 };
 // Validity of the pair (row, col), (row, col+1): in the topography block and
-// not masked out.  Branch-free (masked grids read the mask at a clamped,
-// always valid index) so that a batch of pair loads is not split by waits at
-// control-flow joins.
+// not masked out.  Masked grids read the mask at a clamped, always valid
+// index, so no lane-dependent branch guards the load.
 struct CPair {
   bool c0, c1;
 };
