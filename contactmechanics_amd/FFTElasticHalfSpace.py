@@ -338,18 +338,29 @@ class FreeFFTElasticHalfSpace(PeriodicFFTElasticHalfSpace):
 
     def check_boundaries(self, force=None, tol=0):
         """Raise FreeBoundaryError if the force on the outer ring of the
-        topography exceeds ``tol`` (reference :1217-1269)."""
+        topography exceeds ``tol`` (reference :1217-1269).  For a masked
+        force array, masked pixels are ignored and an edge whose pixels are
+        all masked passes."""
         if force is None:
             force = self.force
-        force = np.asarray(force)
-        nx, ny = self.nb_grid_pts
-        ok = bool((abs(force[:, 0]) <= tol).all())
+        if _is_torch(force):
+            force = force.detach().to("cpu").double().numpy()
+        masked = np.ma.is_masked(force)
+        force = np.ma.asarray(force) if masked else np.asarray(force)
+        nx, ny = self.topography_nb_subdomain_grid_pts
+
+        def edge_ok(vals):
+            small = np.abs(vals) <= tol
+            if masked:
+                return bool(np.ma.getmaskarray(vals).all() or np.ma.filled(small, True).all())
+            return bool(small.all())
+
+        edges = [force[:, 0], force[0, :]]
         if 0 < ny - 1 < force.shape[1]:
-            ok &= bool((abs(force[:, ny - 1]) <= tol).all())
-        ok &= bool((abs(force[0, :]) <= tol).all())
+            edges.append(force[:, ny - 1])
         if 0 < nx - 1 < force.shape[0]:
-            ok &= bool((abs(force[nx - 1, :]) <= tol).all())
-        if not ok:
+            edges.append(force[nx - 1, :])
+        if not all(edge_ok(e) for e in edges):
             raise self.FreeBoundaryError(
                 "The forces not zero at the boundary of the active domain. This is typically an "
                 "indication that the contact geometry exceeds the bounds of the domain. Since this is "

@@ -17,6 +17,11 @@ Extra keywords (not in the reference):
     ``logger`` forces 1 (the host then sees every iteration).
 ``use_graph``
     replay chunks of iterations from a captured hipGraph.
+``refresh_every``
+    FP32 substrates: iterations between FP64 recomputations of the
+    displacements from the forces (0 = the library default, 8).  In between,
+    the displacements (kept as FP64 gaps) are updated by the FP32 transform
+    of the force change, whose round-off shrinks with the change itself.
 
 Substrates built with ``precision="mixed"`` solve in two stages: FP32 bulk
 iterations (fields and transforms FP32, reductions and scalars FP64) to
@@ -35,11 +40,20 @@ from .. import _native
 from ..Tools.Logger import quiet
 
 
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
 def _heights_and_mask(topography):
+    """(heights as FP64 array, valid-pixel mask or None, heights as given).
+    Heights given as a torch tensor (already on the device) stay there."""
     if hasattr(topography, "nb_grid_pts"):
         heights = topography.heights()
     else:
         heights = topography
+    if _is_torch(heights):
+        import torch
+        return heights.to(torch.float64).contiguous(), None, heights
     mask = np.ma.getmask(heights)
     if mask is np.ma.nomask:
         return np.asarray(heights, dtype=np.float64), None, heights
@@ -48,8 +62,33 @@ def _heights_and_mask(topography):
 
 
 def _rms_height(heights):
+    """Area RMS height about the mean of the valid (unmasked) pixels."""
+    if _is_torch(heights):
+        import torch
+        h = heights.to(torch.float64)
+        return float(torch.sqrt(torch.mean((h - h.mean()) ** 2)))
     h = np.ma.compressed(np.ma.asarray(heights))
     return float(np.sqrt(np.mean((h - h.mean()) ** 2)))
+
+
+def default_pentol(topography, heights, nb_grid_pts, offset):
+    """Penetration tolerance used when ``pentol`` is None (reference :150-162):
+    RMS height / (10 mean(nb_grid_pts)); for a flat surface
+    (offset + mean height) / 1000; if that is zero too, 1e-3.  A topography
+    object that has ``rms_height_from_area`` (SurfaceTopography's) supplies
+    the RMS height itself, as in the reference."""
+    if hasattr(topography, "rms_height_from_area"):
+        rms = float(topography.rms_height_from_area())
+    else:
+        rms = _rms_height(heights)
+    pentol = rms / (10 * np.mean(nb_grid_pts))
+    if pentol == 0:
+        n = int(np.prod(nb_grid_pts))
+        total = float(heights.sum()) if _is_torch(heights) else np.sum(np.ma.filled(heights, 0.0))
+        pentol = ((offset if offset is not None else 0) + total / n) / 1000
+    if pentol == 0:
+        pentol = 1e-3
+    return pentol
 
 
 def _to_device_field(arr, substrate, name):
@@ -69,7 +108,7 @@ def constrained_conjugate_gradients(substrate, topography, hardness=None, extern
                                     pentol=None, forcetol=1e-5, thermotol=1e-6, mixfac=0.1,
                                     mixdecfac=0.95, minmixsteps=10, maxiter=100000, logger=quiet,
                                     callback=None, verbose=False, return_device=False,
-                                    iters_per_sync=0, use_graph=True):
+                                    iters_per_sync=0, use_graph=True, refresh_every=0, _it_offset=0):
     """Equilibrium of a rigid ``topography`` pressed on an elastic ``substrate``.
 
     See the reference docstring (``ConstrainedConjugateGradients.py:61-124``);
@@ -98,17 +137,12 @@ def constrained_conjugate_gradients(substrate, topography, hardness=None, extern
 
     h_np, valid, heights = _heights_and_mask(topography)
     nx, ny = substrate.nb_grid_pts
-    if h_np.shape != (nx, ny):
-        raise substrate.Error(f"topography has shape {h_np.shape}, substrate expects {(nx, ny)}")
+    if tuple(h_np.shape) != (nx, ny):
+        raise substrate.Error(f"topography has shape {tuple(h_np.shape)}, substrate expects {(nx, ny)}")
     nb_surface_pts = nx * ny
 
     if pentol is None:  # :150-162
-        pentol = _rms_height(heights) / (10 * np.mean(substrate.nb_grid_pts))
-        if pentol == 0:
-            pentol = ((offset if offset is not None else 0)
-                      + np.sum(np.ma.filled(heights, 0.0)) / nb_surface_pts) / 1000
-        if pentol == 0:
-            pentol = 1e-3
+        pentol = default_pentol(topography, heights, (nx, ny), offset)
     logger.pr(f"maxiter = {maxiter}")
     logger.pr(f"pentol = {pentol}")
 
@@ -121,7 +155,7 @@ def constrained_conjugate_gradients(substrate, topography, hardness=None, extern
     plan = substrate.plan
     dev = substrate._device_str()
     M_shape = tuple(substrate.nb_subdomain_grid_pts)
-    h_dev = torch.from_numpy(h_np).to(dev)
+    h_dev = h_np.to(dev) if _is_torch(h_np) else torch.from_numpy(h_np).to(dev)
     m_dev = None
     if valid is not None:
         m_dev = torch.from_numpy(np.ascontiguousarray(valid, dtype=np.uint8)).to(dev)
@@ -148,31 +182,31 @@ def constrained_conjugate_gradients(substrate, topography, hardness=None, extern
         mixfac=float(mixfac), mixdecfac=float(mixdecfac), minmixsteps=int(minmixsteps),
         maxiter=int(maxiter), init_mode=init_mode,
         iters_per_sync=1 if per_iteration else int(iters_per_sync),
-        use_graph=0 if per_iteration else int(bool(use_graph)))
+        use_graph=0 if per_iteration else int(bool(use_graph)), refresh_every=int(refresh_every))
     stats = _native.CcgStats()
 
     n_surf = int(valid.sum()) if valid is not None else nb_surface_pts
     errors = []
+    it_offset = int(_it_offset)
 
     def progress(_user, st_ptr):
+        # reference :436-492: every iteration that has not converged is logged
+        # (except the one that reaches maxiter) and passed to the callback,
+        # including the maxiter and NaN iterations; the converged one is
+        # reported with the result
         try:
             st = st_ptr.contents
-            if st.status != _native.CM_RUNNING or st.nit == 0:
-                return  # finished: reported below
-            status = _native.DELTA_STR.get(st.last_mode, "?")
-            headers = ["status", "it", "area", "frac. area", "cg area", "total force", "offset"]
-            values = [status, st.nit, int(st.area), st.area / n_surf, int(st.cg_area), st.total_force,
-                      st.offset]
-            if hardness:
-                headers += ["plast. area", "frac.plast. area"]
-                values += [int(st.plastic_area), st.plastic_area / n_surf]
-            logger.st(headers, values)
+            if st.nit == 0 or st.status == _native.CM_CONVERGED:
+                return
+            headers, values = _log_row(st, hardness, verbose, n_surf)
+            if st.nit < maxiter:
+                logger.st(headers, values)
             if callback is not None:
                 d = dict(area=int(st.area), fractional_area=float(st.area / n_surf),
                          rms_penetration=float(st.rms_pen), max_penetration=float(st.max_pen),
                          max_pressure=float(st.max_pres), pad_pressure=float(st.pad_pres),
                          penetration_tol=float(pentol), pressure_tol=float(forcetol))
-                callback(st.nit, f.to(torch.float64).cpu().numpy(), d)
+                callback(st.nit + it_offset, f.to(torch.float64).cpu().numpy(), d)
         except Exception as err:  # re-raised after the solve returns
             errors.append(err)
 
@@ -195,11 +229,35 @@ def constrained_conjugate_gradients(substrate, topography, hardness=None, extern
         _native.check(rc, plan.ctx.handle)
 
     return _assemble_result(substrate, stats, u, f, cmask, hardness, return_device,
-                            logger, n_surf, maxiter)
+                            logger, n_surf, maxiter, verbose)
+
+
+def _log_row(st, hardness, verbose, n_surf):
+    """Column headers and values of one iteration's log line (reference
+    :436-453), from the device statistics."""
+    n_surf = max(n_surf, 1)
+    status = _native.DELTA_STR.get(st.last_mode, "?")
+    headers = ["status", "it", "area", "frac. area", "cg area", "total force", "offset"]
+    values = [status, int(st.nit), int(st.area), st.area / n_surf, int(st.cg_area), st.total_force, st.offset]
+    if hardness:
+        headers += ["plast. area", "frac.plast. area"]
+        values += [int(st.plastic_area), st.plastic_area / n_surf]
+    if verbose:
+        headers += ["rms pen.", "max. pen.", "max. force", "max. pad force", "max. du", "CG area",
+                    "frac. CG area", "sum(nc_r)"]
+        values += [st.rms_pen, st.max_pen, st.max_pres, st.pad_pres, st.maxdu, int(st.cg_area),
+                   st.cg_area / n_surf, int(st.nc_count)]
+        if status == "mix":
+            headers += ["mixfac"]
+            values += [st.mixfac]
+        else:
+            headers += ["tau"]
+            values += [st.tau]
+    return headers, values
 
 
 def _assemble_result(substrate, stats, u, f, cmask, hardness, return_device, logger, n_surf,
-                     maxiter):
+                     maxiter, verbose=False):
     import torch
     nx, ny = substrate.nb_grid_pts
     result = optim.OptimizeResult()
@@ -225,9 +283,8 @@ def _assemble_result(substrate, stats, u, f, cmask, hardness, return_device, log
         result.active_set = active.cpu().numpy()
         if converged and hardness is not None:  # :464-467
             result.plastic = gneg[:nx, :ny].cpu().numpy()
-    headers = ["status", "it", "area", "frac. area", "cg area", "total force", "offset"]
-    values = ["CONVERGED" if converged else "NOT CONVERGED", result.nit, int(stats.area),
-              stats.area / max(n_surf, 1), int(stats.cg_area), stats.total_force, stats.offset]
+    headers, values = _log_row(stats, hardness, verbose, n_surf)
+    values[0] = "CONVERGED" if converged else "NOT CONVERGED"
     logger.st(headers, values, force_print=True)
     result.success = bool(converged)
     if converged:
@@ -250,7 +307,7 @@ def _mixed_precision(substrate, topography, pentol=None, forcetol=1e-5, thermoto
     h_np, valid, heights = _heights_and_mask(topography)
     nx, ny = substrate.nb_grid_pts
     if pentol is None:
-        pentol = _rms_height(heights) / (10 * np.mean(substrate.nb_grid_pts))
+        pentol = default_pentol(topography, heights, (nx, ny), kw.get("offset"))
     rms = _rms_height(heights)
     scale_f = abs(external_force) / (nx * ny) if external_force is not None else 0.0
     twin = substrate.fp32_twin()
@@ -264,12 +321,16 @@ def _mixed_precision(substrate, topography, pentol=None, forcetol=1e-5, thermoto
     f0 = torch.zeros(tuple(substrate.nb_subdomain_grid_pts), dtype=torch.float64, device=substrate._device_str())
     f0[:nx, :ny] = -r32.jac.to(torch.float64)
     kw.pop("initial_displacements", None)
+    # the FP64 stage continues the iteration count: its callback and log
+    # lines are numbered from nit_fp32 + 1, and its budget is what is left
     r64 = constrained_conjugate_gradients(
         substrate, topography, external_force=external_force, pentol=pentol, forcetol=forcetol,
         thermotol=thermotol, maxiter=max(1, maxiter - int(r32.nit)), initial_forces=f0,
-        return_device=return_device, **kw)
+        return_device=return_device, _it_offset=int(r32.nit), **kw)
     r64.nit_fp32 = int(r32.nit)
     r64.nit_fp64 = int(r64.nit)
     r64.nit = r64.nit_fp32 + r64.nit_fp64
     r64.nfev = int(r64.nfev) + int(r32.nfev)
+    if not r64.success:
+        r64.message = "Reached maxiter = {}".format(maxiter)
     return r64

@@ -26,7 +26,7 @@ class Logger:
         for v in values:
             if isinstance(v, str):
                 cols.append(f"{v:>22}")
-            elif isinstance(v, (int,)) or (hasattr(v, "is_integer") and False):
+            elif isinstance(v, int):
                 cols.append(f"{v:>22d}")
             else:
                 cols.append(f"{float(v):>22.12e}")

@@ -6,8 +6,9 @@ The reference reads from SurfaceTopography (absent here): ``heights()``,
 (``ConstrainedConjugateGradients.py:141-160``) and, for plasticity,
 ``PlasticTopography.hardness`` / ``plastic_displ`` / ``plastic_area``
 (``PlasticSystemSpecialisations.py:65-68``).  Only those members are provided.
-``rms_height_from_area`` is the area RMS about the mean (parity unpinned: the
-original lives in SurfaceTopography).
+``rms_height_from_area`` is the area RMS of the valid heights about their mean
+(SurfaceTopography's definition for uniform grids); a SurfaceTopography object
+passed to the solver supplies its own.
 """
 
 import numpy as np

@@ -60,7 +60,7 @@ class CcgParams(ctypes.Structure):
                 ("mixfac", ctypes.c_double), ("mixdecfac", ctypes.c_double),
                 ("minmixsteps", ctypes.c_int), ("maxiter", ctypes.c_int),
                 ("init_mode", ctypes.c_int), ("iters_per_sync", ctypes.c_int),
-                ("use_graph", ctypes.c_int)]
+                ("use_graph", ctypes.c_int), ("refresh_every", ctypes.c_int)]
 
 
 class CcgStats(ctypes.Structure):

@@ -168,8 +168,15 @@ struct LinePlan {
   int tpl;        // threads per line
   int E;          // most elements a thread holds in any pass
   int npass;      // number of radix passes
-  int rlast;      // radix[npass - 1] (read without indexing the array at run time)
-  int radix[14];  // radix of each pass (product == L)
+  int rlast;      // radix of the last pass
+  // radix of pass s, minus one, in bits 4s..4s+3 (radices 2..16, at most 16
+  // passes): a shift instead of an indexed array, which a kernel would have
+  // to copy to private memory to index at run time
+  uint64_t rcode;
+  __host__ __device__ __forceinline__ int radix(int s) const { return (int)((rcode >> (4 * s)) & 15u) + 1; }
+  __host__ void set_radix(int s, int R) {
+    rcode = (rcode & ~((uint64_t)15u << (4 * s))) | ((uint64_t)(R - 1) << (4 * s));
+  }
 };
 
 // Twiddle table: tw[k] = exp(-2 pi i k / L), k in [0, L).
@@ -298,7 +305,7 @@ __device__ void fft_line(cplx<T>* lds, int base, int stride, const LinePlan& p, 
                          const cplx<T>* __restrict__ tw) {
   int Ns = 1;
   for (int s = 0; s < p.npass; ++s) {
-    const int R = p.radix[s];
+    const int R = p.radix(s);
     const int nb = active ? nbutterflies(p.L, R, tl, p.tpl) : 0;
     pass_dispatch<T, INV, EMAX, EARLY>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw);
     Ns *= R;
@@ -373,7 +380,7 @@ __device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const
                            const cplx<T>* __restrict__ tw) {
   int s0 = 0, s1 = p.npass, Ns = 1;
   if (IN_REGS) {
-    switch (p.radix[0]) {
+    switch (p.radix(0)) {
       case 2: first_pass_regs<T, 2, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
       case 3: first_pass_regs<T, 3, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
       case 4: first_pass_regs<T, 4, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
@@ -384,12 +391,12 @@ __device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const
       default: first_pass_regs<T, 8, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
     }
     __syncthreads();
-    Ns = p.radix[0];
+    Ns = p.radix(0);
     s0 = 1;
   }
   if (OUT_REGS) s1 = p.npass - 1;
   for (int s = s0; s < s1; ++s) {
-    const int R = p.radix[s];
+    const int R = p.radix(s);
     const int nb = nbutterflies(p.L, R, tl, p.tpl);
     pass_dispatch<T, INV, EMAX, EARLY>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw);
     Ns *= R;
@@ -397,7 +404,7 @@ __device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const
   if (OUT_REGS) {
     if (IN_REGS && p.npass == 1) {
       // the single pass already wrote natural order to LDS
-      const int R = p.radix[0];
+      const int R = p.radix(0);
       const int nb = nbutterflies(p.L, R, tl, p.tpl);
 #pragma unroll
       for (int e = 0; e < EMAX; ++e)
@@ -425,7 +432,7 @@ __device__ __forceinline__ void fft_line_to_last(cplx<T>* lds, int base, int str
                                  const cplx<T>* __restrict__ tw) {
   int Ns = 1;
   for (int s = 0; s + 1 < p.npass; ++s) {
-    const int R = p.radix[s];
+    const int R = p.radix(s);
     pass_dispatch<T, INV, EMAX, EARLY>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nbutterflies(p.L, R, tl, p.tpl), tw);
     Ns *= R;
   }

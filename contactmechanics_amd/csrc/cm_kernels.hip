@@ -65,24 +65,53 @@ struct Scal {
   int has_last_tf, has_last_mh, use_beta, do_nc, uniform, pad_;
 };
 
+// One instance of a solve (or of a plain operator evaluation).  T is the
+// precision of the plan: the caller's displacement and force buffers, the
+// conjugate direction and the transforms of the iteration.  During a solve
+// the forces and the gap are kept in FP64 (fm, w); FP64 plans keep them in
+// the caller's buffers themselves.
 template <typename T>
 struct Inst {
-  T* u;            // displacements (M grid); during the iterations of a solve
-                   // u - h on the computational points (see ElemPhaseA)
-  T* f;            // forces (M grid)
+  T* u;            // displacements (M grid), the caller's buffer
+  double* w;       // during a solve: the gap u - h on the computational
+                   // points, u elsewhere (see ElemPhaseA); FP64 plans: == u
+  T* f;            // forces (M grid), the caller's buffer
+  double* fm;      // during a solve: the forces in FP64; FP64 plans: == f
   T* t;            // conjugate direction (M grid)
   uint8_t* cm;     // bit0 contact set c, bit1 gap<0 at start of iteration
-  const T* h;      // heights (nx*ny), in the plan precision
+  const double* h; // heights (nx*ny)
   const uint8_t* surf;  // valid-pixel mask (nx*ny) or null
   cplx<T>* S;      // spectrum work buffer
   const T* G;      // evaluate_disp kernel, (N2+1) x Mx
   const T* K;      // evaluate_force kernel, (N2+1) x Mx
+  // FP64 transforms of an FP32 plan (set-up and the periodic displacement
+  // refresh); FP64 plans: the same buffers as S, G, K
+  cplx<double>* S64;
+  const double* G64;
+  const double* K64;
   const T* src;    // operator input (plain evaluate)
   T* dst;          // operator output (plain evaluate)
   Scal* sc;
   double* part;    // per-block partial reductions
   unsigned* ctr;   // last-block tickets (8 slots of kCtrSlot counters)
 };
+
+// Spectrum buffer and Green's function for transforms in precision TT of an
+// instance with fields in precision TF.
+template <typename TT, typename TF>
+__device__ __forceinline__ cplx<TT>* spec_of(const Inst<TF>& in) {
+  if constexpr (std::is_same<TT, TF>::value)
+    return in.S;
+  else
+    return in.S64;
+}
+template <typename TT, typename TF>
+__device__ __forceinline__ const TT* green_of(const Inst<TF>& in, int useK) {
+  if constexpr (std::is_same<TT, TF>::value)
+    return useK ? in.K : in.G;
+  else
+    return useK ? in.K64 : in.G64;
+}
 
 template <typename T>
 struct InstArr {
@@ -111,6 +140,9 @@ __device__ __forceinline__ void acquire_agent() { __builtin_amdgcn_fence(__ATOMI
 constexpr int kTicketGroups = 16;
 constexpr int kCtrLine = 16;                             // unsigned per 64-byte line
 constexpr int kCtrSlot = kCtrLine * (kTicketGroups + 1); // counters of one reduction slot
+
+// FP32 plans: default number of iterations per FP64 displacement refresh.
+constexpr int kRefreshEvery = 8;
 
 // Cross-block reduction: every block reduces NS slots, writes its partials,
 // takes a ticket; the block drawing the last ticket reduces all partials in a
@@ -355,10 +387,11 @@ __host__ __device__ constexpr int fft_block_max(int) { return 512; }
 // Row pass: real rows -> packed half spectrum (forward, unnormalised).
 template <typename T, int EMAX, class P>
 __global__ void __launch_bounds__(fft_block_max(EMAX)) k_r2c(Geom g, LinePlan lp, const cplx<T>* __restrict__ tw,
-                                             const cplx<T>* __restrict__ twp, InstArr<T> ia, int ctr_slot) {
+                                             const cplx<T>* __restrict__ twp, InstArr<typename P::F> ia,
+                                             int ctr_slot) {
   extern __shared__ __align__(16) unsigned char smem_r2c[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_r2c);
-  const Inst<T>& in = ia.get(blockIdx.y);
+  const Inst<typename P::F>& in = ia.get(blockIdx.y);
   P p(in);
   if (p.skip()) return;
   const int N2 = g.N2;
@@ -372,7 +405,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_r2c(Geom g, LinePlan lp
   // the producer feeds the first FFT pass directly (first-pass order)
   cplx<T> v[EMAX];
   {
-    const int R0 = lp.radix[0];
+    const int R0 = lp.radix(0);
     const int nb0 = nbutterflies(N2, R0, tl, lp.tpl);
 #pragma unroll
     for (int e0 = 0; e0 < EMAX; e0 += kPairs) {
@@ -412,7 +445,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_r2c(Geom g, LinePlan lp
     }
   }
   __syncthreads();
-  cplx<T>* S = in.S;
+  cplx<T>* S = spec_of<T>(in);
   const int tot = g.R * N2;
   const RowBlockMap map(g, blockIdx.x);
   const GViewW<T> Sg(S);
@@ -446,18 +479,19 @@ struct TauRed {
 // only; 3: forward only, then sum_k G(k) |T(k)|^2 over the whole spectrum and
 // the step length tau from it (no output).
 // skip: 0 never, 1 when the solve is done, 2 when done or in a mixing step.
-template <typename T, int EMAX, int MODE>
+// T: transform precision; TF: field precision of the instances.
+template <typename T, int EMAX, int MODE, typename TF>
 __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp, LinePlan rv, const cplx<T>* __restrict__ tw,
-                                              InstArr<T> ia, int useK, int skip) {
+                                              InstArr<TF> ia, int useK, int skip) {
   extern __shared__ __align__(16) unsigned char smem_col[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_col);
-  const Inst<T>& in = ia.get(blockIdx.y);
+  const Inst<TF>& in = ia.get(blockIdx.y);
   if (skip && in.sc) {
     if (in.sc->done) return;
     if (skip == 2 && in.sc->mode == MODE_MIX) return;
   }
-  cplx<T>* S = in.S;
-  const T* G = useK ? in.K : in.G;
+  cplx<T>* S = spec_of<T>(in);
+  const T* G = green_of<T>(in, useK);
   const int cb = blockIdx.x;
   const int Mx = g.Mx, Cb = g.Cb;
   const size_t base = (size_t)cb * Mx * Cb;
@@ -471,7 +505,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
     double acc = 0.0;
     const int c = threadIdx.x % Cb, tl = threadIdx.x / Cb;
     if (cb != 0) {
-      const int R0 = lp.radix[0], Rl = lp.rlast;
+      const int R0 = lp.radix(0), Rl = lp.rlast;
       const int nb0 = nbutterflies(Mx, R0, tl, lp.tpl), nbl = nbutterflies(Mx, Rl, tl, lp.tpl);
       cplx<T> v[EMAX];
       const GView<T> Sg(S);
@@ -519,7 +553,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
     // forward pass -> x G in registers -> first inverse pass (same radix, same
     // slots), last inverse pass -> HBM.  Two LDS round trips fewer each way.
     const int c = threadIdx.x % Cb, tl = threadIdx.x / Cb;
-    const int R0 = lp.radix[0], Rl = lp.rlast;
+    const int R0 = lp.radix(0), Rl = lp.rlast;
     const int nb0 = nbutterflies(Mx, R0, tl, lp.tpl), nbl = nbutterflies(Mx, Rl, tl, lp.tpl);
     cplx<T> v[EMAX];
     const GView<T> Sg(S);
@@ -604,14 +638,15 @@ __device__ __forceinline__ void c2r_finish(const P& p, const Geom& g, int row, i
 // round trip r2c -> c2r scales by Mx*My, which the kernels G/K absorb).
 template <typename T, int EMAX, class P>
 __global__ void __launch_bounds__(fft_block_max(EMAX)) k_c2r(Geom g, LinePlan lp, const cplx<T>* __restrict__ tw,
-                                             const cplx<T>* __restrict__ twp, InstArr<T> ia, int ctr_slot) {
+                                             const cplx<T>* __restrict__ twp, InstArr<typename P::F> ia,
+                                             int ctr_slot) {
   extern __shared__ __align__(16) unsigned char smem_c2r[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_c2r);
-  const Inst<T>& in = ia.get(blockIdx.y);
+  const Inst<typename P::F>& in = ia.get(blockIdx.y);
   P p(in);
   if (p.skip()) return;
   const int N2 = g.N2;
-  const cplx<T>* S = in.S;
+  const cplx<T>* S = spec_of<T>(in);
   const int tot = g.R * N2;
   const RowBlockMap map(g, blockIdx.x);
   const GView<T> Sg(S);
@@ -721,40 +756,43 @@ __global__ void __launch_bounds__(256) k_elem(Geom g, InstArr<T> ia, int ctr_slo
 }
 
 // ---------------------------------------------------------------------------
-// Functors for plain operator evaluation
+// Functors for plain operator evaluation.  TF: field precision of the
+// instance, TT: precision of the transform they feed or consume.
 
-template <typename T>
+template <typename TF, typename TT = TF>
 struct ProdCopy {  // input row from src
+  using F = TF;
   static constexpr int NS = 0;
   __device__ static constexpr int op(int) { return RSUM; }
   struct Rec {
-    P2<T> v;
+    P2<TF> v;
   };
-  const Inst<T>& in;
-  __device__ ProdCopy(const Inst<T>& i) : in(i) {}
+  const Inst<TF>& in;
+  __device__ ProdCopy(const Inst<TF>& i) : in(i) {}
   __device__ bool skip() const { return false; }
   __device__ Rec fetch(const Geom& g, int row, int col) const { return Rec{ld2(in.src + (size_t)row * g.My + col)}; }
-  __device__ void make(const Geom&, int, int, const Rec& r, T& a, T& b, double*) const {
-    a = r.v.a;
-    b = r.v.b;
+  __device__ void make(const Geom&, int, int, const Rec& r, TT& a, TT& b, double*) const {
+    a = (TT)r.v.a;
+    b = (TT)r.v.b;
   }
   __device__ void finalize(double*) const {}
 };
 
-template <typename T>
+template <typename TF, typename TT = TF>
 struct ConsCopy {  // output row to dst
+  using F = TF;
   static constexpr int NS = 0;
   static constexpr int kBatch = kPairs;
   __device__ static constexpr int op(int) { return RSUM; }
   struct Rec {};
-  const Inst<T>& in;
-  __device__ ConsCopy(const Inst<T>& i) : in(i) {}
+  const Inst<TF>& in;
+  __device__ ConsCopy(const Inst<TF>& i) : in(i) {}
   __device__ bool skip() const { return false; }
   __device__ Rec fetch(const Geom&, int, int) const { return Rec{}; }
   struct Ctx {};
   __device__ Ctx ctx() const { return Ctx{}; }
-  __device__ void take(const Geom& g, int row, int col, const Rec&, T a, T b, double*, const Ctx&) const {
-    st2(in.dst + (size_t)row * g.My + col, a, b);
+  __device__ void take(const Geom& g, int row, int col, const Rec&, TT a, TT b, double*, const Ctx&) const {
+    st2(in.dst + (size_t)row * g.My + col, (TF)a, (TF)b);
   }
   __device__ void finalize(double*) const {}
 };
@@ -765,6 +803,7 @@ struct ConsCopy {  // output row to dst
 // u[comp] = max(u, h + offset) (:201); max over valid heights (:196-197).
 template <typename T>
 struct ElemInit {
+  using F = T;
   static constexpr int NS = 2;  // max_h, n_comp
   __device__ static constexpr int op(int i) { return i == 0 ? RMAX : RSUM; }
   struct Rec {
@@ -802,43 +841,58 @@ struct ElemInit {
   }
 };
 
-// f = evaluate_force(u) output with f[pad] = 0 (:212, :219).
-template <typename T>
+// f = evaluate_force(u) output with f[pad] = 0 (:212, :219), into the FP64
+// forces of the solve.
+template <typename TF, typename TT>
 struct ConsInitForce {
+  using F = TF;
   static constexpr int NS = 0;
   static constexpr int kBatch = kPairs;
   __device__ static constexpr int op(int) { return RSUM; }
   struct Rec {
-    HPair h;
+    CPair c;
   };
-  const Inst<T>& in;
-  __device__ ConsInitForce(const Inst<T>& i) : in(i) {}
+  const Inst<TF>& in;
+  __device__ ConsInitForce(const Inst<TF>& i) : in(i) {}
   __device__ bool skip() const { return false; }
-  __device__ Rec fetch(const Geom& g, int row, int col) const { return Rec{ldh(g, in.h, in.surf, row, col)}; }
+  __device__ Rec fetch(const Geom& g, int row, int col) const { return Rec{ldc(g, in.surf, row, col)}; }
   struct Ctx {};
   __device__ Ctx ctx() const { return Ctx{}; }
-  __device__ void take(const Geom& g, int row, int col, const Rec& r, T a, T b, double*, const Ctx&) const {
-    st2(in.f + (size_t)row * g.My + col, r.h.c0 ? a : T(0), r.h.c1 ? b : T(0));
+  __device__ void take(const Geom& g, int row, int col, const Rec& r, TT a, TT b, double*, const Ctx&) const {
+    st2(in.fm + (size_t)row * g.My + col, r.c.c0 ? (double)a : 0.0, r.c.c1 ? (double)b : 0.0);
   }
   __device__ void finalize(double*) const {}
 };
 
-// f[pad] = 0 after u = evaluate_disp(initial_forces) (:213-219).
+// The initial forces of the solve with f[pad] = 0 after u =
+// evaluate_disp(initial_forces) (:213-219): FP64 plans zero the padding in
+// place, FP32 plans copy the forces into their FP64 forces.
 template <typename T>
 struct ElemZeroPad {
+  using F = T;
   static constexpr int NS = 0;
   __device__ static constexpr int op(int) { return RSUM; }
   struct Rec {
-    HPair h;
+    CPair c;
+    P2<T> f;
   };
   const Inst<T>& in;
   __device__ ElemZeroPad(const Inst<T>& i) : in(i) {}
   __device__ bool skip() const { return false; }
-  __device__ Rec fetch(const Geom& g, int row, int col) const { return Rec{ldh(g, in.h, in.surf, row, col)}; }
+  __device__ Rec fetch(const Geom& g, int row, int col) const {
+    Rec r;
+    r.c = ldc(g, in.surf, row, col);
+    if (!std::is_same<T, double>::value) r.f = ld2(in.f + (size_t)row * g.My + col);
+    return r;
+  }
   __device__ void emit(const Geom& g, int row, int col, const Rec& r, double*) const {
-    T* fp = in.f + (size_t)row * g.My + col;
-    if (!r.h.c0) fp[0] = T(0);
-    if (!r.h.c1) fp[1] = T(0);
+    double* fp = in.fm + (size_t)row * g.My + col;
+    if (std::is_same<T, double>::value) {
+      if (!r.c.c0) fp[0] = 0.0;
+      if (!r.c.c1) fp[1] = 0.0;
+    } else {
+      st2(fp, r.c.c0 ? (double)r.f.a : 0.0, r.c.c1 ? (double)r.f.b : 0.0);
+    }
   }
   __device__ void finalize(double*) const {}
 };
@@ -900,18 +954,20 @@ constexpr uint8_t kCur = 1, kGneg = 2, kNext = 4;
 
 // Phase A of the first iteration (later iterations fuse it into ConsDisp):
 // contact set and gap statistics (:238-273, :312-318 brackets).  It also
-// replaces u by the gap u - h (:265) on the computational points: every later
-// pass of the iteration uses only the gap, so the heights are read once per
-// iteration (by ConsDisp, which stores u_new - h); ElemFinish restores u.
-// The gap is the one the oracle forms, fl(u - h), rounded to the field type.
-// It runs also for an instance that is already done (maxiter < 1), so that
+// replaces u by the gap u - h (:265) on the computational points, in FP64 in
+// the gap buffer w: every later pass of the iteration uses only the gap, so
+// the heights are read at most once per iteration; ElemFinish restores u.
+// FP64 plans form the gap the oracle forms, fl(u - h), in place.  It runs
+// also for an instance that is already done (maxiter < 1), so that
 // ElemFinish always finds the gap.
 template <typename T>
 struct ElemPhaseA {
+  using F = T;
   static constexpr int NS = 6;  // A_contact, A_cg, Sd, Sdd, gmax, gmin
   __device__ static constexpr int op(int i) { return i == 4 ? RMAX : (i == 5 ? RMIN : RSUM); }
   struct Rec {
-    P2<T> f, u;
+    P2<double> f;
+    P2<T> u;
     HPair h;
   };
   const Inst<T>& in;
@@ -919,17 +975,18 @@ struct ElemPhaseA {
   __device__ bool skip() const { return false; }
   __device__ Rec fetch(const Geom& g, int row, int col) const {
     const size_t p = (size_t)row * g.My + col;
-    return Rec{ld2(in.f + p), ld2(in.u + p), ldh(g, in.h, in.surf, row, col)};
+    return Rec{ld2(in.fm + p), ld2(in.u + p), ldh(g, in.h, in.surf, row, col)};
   }
   __device__ void emit(const Geom& g, int row, int col, const Rec& r, double* acc) const {
     const Scal* s = in.sc;
-    const T w0 = r.h.c0 ? (T)((double)r.u.a - r.h.h0) : r.u.a;
-    const T w1 = r.h.c1 ? (T)((double)r.u.b - r.h.h1) : r.u.b;
-    const bool c0 = phase_a_point(s->has_H, s->H, s->shift, (double)r.f.a, (double)w0, r.h.c0, acc);
-    const bool c1 = phase_a_point(s->has_H, s->H, s->shift, (double)r.f.b, (double)w1, r.h.c1, acc);
+    const double w0 = r.h.c0 ? (double)r.u.a - r.h.h0 : (double)r.u.a;
+    const double w1 = r.h.c1 ? (double)r.u.b - r.h.h1 : (double)r.u.b;
+    const bool c0 = phase_a_point(s->has_H, s->H, s->shift, r.f.a, w0, r.h.c0, acc);
+    const bool c1 = phase_a_point(s->has_H, s->H, s->shift, r.f.b, w1, r.h.c1, acc);
     const size_t p = (size_t)row * g.My + col;
     stm(in.cm + p, c0 ? kNext : 0, c1 ? kNext : 0);
-    if (r.h.c0 || r.h.c1) st2(in.u + p, w0, w1);
+    // FP64 plans write only where the gap differs from u (w aliases u)
+    if (!std::is_same<T, double>::value || r.h.c0 || r.h.c1) st2(in.w + p, w0, w1);
   }
   __device__ void finalize(double* a) const {
     if (!in.sc->done) phase_a_logic(in.sc, a);
@@ -953,7 +1010,7 @@ __global__ void __launch_bounds__(1024) k_bisect(Geom g, InstArr<T> ia) {
     for (size_t q = threadIdx.x; q < Mt; q += blockDim.x) {
       const int i = (int)(q / g.ny), j = (int)(q - (size_t)i * g.ny);
       if (in.surf && !in.surf[q]) continue;
-      const double g0 = (double)in.u[(size_t)i * g.My + j];  // u holds the gap u - h
+      const double g0 = in.w[(size_t)i * g.My + j];  // the gap u - h
       v += (g0 - x < 0.0) ? 1.0 : 0.0;
     }
     int ops[1] = {RSUM};
@@ -1005,10 +1062,12 @@ __global__ void __launch_bounds__(1024) k_bisect(Geom g, InstArr<T> ia) {
 // first forward transform; partial sum_c g t for tau (:292).
 template <typename T>
 struct ProdDir {
+  using F = T;
   static constexpr int NS = 1;
   __device__ static constexpr int op(int) { return RSUM; }
   struct Rec {
-    P2<T> w, t;
+    P2<double> w;
+    P2<T> t;
     P2<uint8_t> cm;
     CPair cp;
   };
@@ -1018,7 +1077,7 @@ struct ProdDir {
   __device__ bool skip() const { return sv.done || sv.mode == MODE_MIX; }
   __device__ Rec fetch(const Geom& g, int row, int col) const {
     const size_t p = (size_t)row * g.My + col;
-    return Rec{ld2(in.u + p), ld2(in.t + p), ldm(in.cm + p), ldc(g, in.surf, row, col)};
+    return Rec{ld2(in.w + p), ld2(in.t + p), ldm(in.cm + p), ldc(g, in.surf, row, col)};
   }
   // wv: the stored gap u - h
   __device__ T one(const Scal* s, bool comp, bool c, double wv, T told, double* acc) const {
@@ -1032,8 +1091,8 @@ struct ProdDir {
   }
   __device__ void make(const Geom& g, int row, int col, const Rec& r, T& a, T& b, double* acc) const {
     const Scal* s = &sv;
-    a = one(s, r.cp.c0, r.cm.a & kNext, (double)r.w.a, r.t.a, acc);
-    b = one(s, r.cp.c1, r.cm.b & kNext, (double)r.w.b, r.t.b, acc);
+    a = one(s, r.cp.c0, r.cm.a & kNext, r.w.a, r.t.a, acc);
+    b = one(s, r.cp.c1, r.cm.b & kNext, r.w.b, r.t.b, acc);
     st2(in.t + (size_t)row * g.My + col, a, b);
   }
   __device__ void finalize(double* a) const { in.sc->Scgt = a[0]; }
@@ -1062,12 +1121,15 @@ __device__ __forceinline__ double force_update(const Scal* s, double fv, double 
 // Phase B: projection statistics (:332-389).
 template <typename T>
 struct ElemPhaseB {
+  using F = T;
   static constexpr int NS = 7;  // n_tensile, max_tensile, pad_max, A_fl, min_fl, sum_f2, nc
   __device__ static constexpr int op(int i) {
     return (i == 1 || i == 2) ? RMAX : (i == 4 ? RMIN : RSUM);
   }
   struct Rec {
-    P2<T> f, t, w;
+    P2<double> f;
+    P2<T> t;
+    P2<double> w;
     P2<uint8_t> cm;
     CPair cp;
   };
@@ -1077,7 +1139,7 @@ struct ElemPhaseB {
   __device__ bool skip() const { return sv.done != 0; }
   __device__ Rec fetch(const Geom& g, int row, int col) const {
     const size_t p = (size_t)row * g.My + col;
-    return Rec{ld2(in.f + p), ld2(in.t + p), ld2(in.u + p), ldm(in.cm + p), ldc(g, in.surf, row, col)};
+    return Rec{ld2(in.fm + p), ld2(in.t + p), ld2(in.w + p), ldm(in.cm + p), ldc(g, in.surf, row, col)};
   }
   // wv: the stored gap u - h
   __device__ uint8_t one(const Scal* s, bool comp, uint8_t cmv, double fv, double tv, double wv,
@@ -1107,8 +1169,8 @@ struct ElemPhaseB {
   }
   __device__ void emit(const Geom& g, int row, int col, const Rec& r, double* acc) const {
     const Scal* s = &sv;
-    const uint8_t c0 = one(s, r.cp.c0, r.cm.a, (double)r.f.a, (double)r.t.a, (double)r.w.a, acc);
-    const uint8_t c1 = one(s, r.cp.c1, r.cm.b, (double)r.f.b, (double)r.t.b, (double)r.w.b, acc);
+    const uint8_t c0 = one(s, r.cp.c0, r.cm.a, r.f.a, (double)r.t.a, r.w.a, acc);
+    const uint8_t c1 = one(s, r.cp.c1, r.cm.b, r.f.b, (double)r.t.b, r.w.b, acc);
     if (s->mode == MODE_MIX) stm(in.cm + (size_t)row * g.My + col, c0, c1);
   }
   __device__ void finalize(double* a) const {
@@ -1146,12 +1208,18 @@ struct ElemPhaseB {
 };
 
 // Final projected force (:359-384) fused into the second forward transform.
-template <typename T>
+// The transform (precision TT) takes the new force (INC = false) or, for the
+// incremental displacement update of FP32 plans, the change of the force
+// f_new - f_old (INC = true), formed in FP64.
+template <typename T, typename TT, bool INC>
 struct ProdForce {
+  using F = T;
   static constexpr int NS = 1;  // sum f over comp
   __device__ static constexpr int op(int) { return RSUM; }
   struct Rec {
-    P2<T> f, t, w;
+    P2<double> f;
+    P2<T> t;
+    P2<double> w;
     P2<uint8_t> cm;
     CPair cp;
   };
@@ -1161,11 +1229,11 @@ struct ProdForce {
   __device__ bool skip() const { return sv.done != 0; }
   __device__ Rec fetch(const Geom& g, int row, int col) const {
     const size_t p = (size_t)row * g.My + col;
-    return Rec{ld2(in.f + p), ld2(in.t + p), ld2(in.u + p), ldm(in.cm + p), ldc(g, in.surf, row, col)};
+    return Rec{ld2(in.fm + p), ld2(in.t + p), ld2(in.w + p), ldm(in.cm + p), ldc(g, in.surf, row, col)};
   }
   // wv: the stored gap u - h
-  __device__ T one(const Scal* s, bool comp, uint8_t& cmv, double fv, double tv, double wv,
-                   double* acc) const {
+  __device__ double one(const Scal* s, bool comp, uint8_t& cmv, double fv, double tv, double wv,
+                        double* acc) const {
     double gap = 0.0;
     if (comp) gap = wv - s->offset;
     const bool c = (cmv & kNext) != 0;
@@ -1185,18 +1253,24 @@ struct ProdForce {
       if (s->do_nc && nc) fo = fo + s->tau * gap;         // :384
       cmv = (uint8_t)((c ? kCur | kNext : 0) | (gap < 0.0 ? kGneg : 0));
     }
-    const T ft = (T)fo;
-    if (comp) acc[0] += (double)ft;
-    return ft;
+    if (comp) acc[0] += fo;
+    return fo;
   }
-  __device__ void make(const Geom& g, int row, int col, const Rec& r, T& a, T& b, double* acc) const {
+  __device__ void make(const Geom& g, int row, int col, const Rec& r, TT& a, TT& b, double* acc) const {
     const Scal* s = &sv;
     uint8_t m0 = r.cm.a, m1 = r.cm.b;
-    a = one(s, r.cp.c0, m0, (double)r.f.a, (double)r.t.a, (double)r.w.a, acc);
-    b = one(s, r.cp.c1, m1, (double)r.f.b, (double)r.t.b, (double)r.w.b, acc);
+    const double f0 = one(s, r.cp.c0, m0, r.f.a, (double)r.t.a, r.w.a, acc);
+    const double f1 = one(s, r.cp.c1, m1, r.f.b, (double)r.t.b, r.w.b, acc);
     const size_t p = (size_t)row * g.My + col;
-    st2(in.f + p, a, b);
+    st2(in.fm + p, f0, f1);
     if (r.cp.c0 || r.cp.c1) stm(in.cm + p, m0, m1);
+    if (INC) {
+      a = (TT)(f0 - r.f.a);
+      b = (TT)(f1 - r.f.b);
+    } else {
+      a = (TT)f0;
+      b = (TT)f1;
+    }
   }
   __device__ void finalize(double* a) const { in.sc->total_force = -a[0]; }
 };
@@ -1205,12 +1279,17 @@ struct ProdForce {
 // convergence tests (:391-434), NaN check (:491-492), maxiter (:494-509);
 // fused with the start of the next iteration (contact set from the final
 // forces, shifted gap sums of u_new), so no separate phase-A pass is needed.
-// The displacement buffer keeps the gap u - h on the computational points:
-// the old gap is read, u_new - h stored; maxdu and max_pen are formed from
-// the gaps (u_new - u_old = gap_new - gap_old, h + offset - u_new =
-// offset - gap_new), which is the only place the heights are read.
-template <typename T>
+// The gap buffer keeps u - h on the computational points (FP64): the old
+// gap is read and the new one stored; maxdu and max_pen are formed from the
+// gaps (u_new - u_old = gap_new - gap_old, h + offset - u_new = offset -
+// gap_new).  INC = false: the transform gives u_new and the heights are read
+// (gap_new = u_new - h).  INC = true (FP32 plans between FP64 refreshes):
+// the transform gives evaluate_disp(f_new - f_old) and gap_new = gap_old +
+// that change, so FP32 round-off enters only through the change of the
+// displacements, not through the displacements themselves.
+template <typename T, typename TT, bool INC>
 struct ConsDisp {
+  using F = T;
   static constexpr int NS = 8;  // maxdu, max_pen | A_contact, A_cg, Sd, Sdd, gmax, gmin (next iteration)
   // two output pairs per batch: the record by value and the transform's
   // registers leave no room for four under 128 VGPRs (four waves per SIMD)
@@ -1219,7 +1298,7 @@ struct ConsDisp {
     return i < 2 ? RMAX : (i == 6 ? RMAX : (i == 7 ? RMIN : RSUM));
   }
   struct Rec {
-    P2<T> u, f;
+    P2<double> w, f;
     P2<uint8_t> cm;
     HPair h;
   };
@@ -1228,7 +1307,17 @@ struct ConsDisp {
   __device__ bool skip() const { return in.sc->done != 0; }
   __device__ Rec fetch(const Geom& g, int row, int col) const {
     const size_t p = (size_t)row * g.My + col;
-    return Rec{ld2(in.u + p), ld2(in.f + p), ldm(in.cm + p), ldh(g, in.h, in.surf, row, col)};
+    Rec r;
+    r.w = ld2(in.w + p);
+    r.f = ld2(in.fm + p);
+    r.cm = ldm(in.cm + p);
+    if (INC) {
+      const CPair c = ldc(g, in.surf, row, col);
+      r.h = HPair{0.0, 0.0, c.c0, c.c1};
+    } else {
+      r.h = ldh(g, in.h, in.surf, row, col);
+    }
+    return r;
   }
   // The scalars the consumer needs, read once per thread just before the
   // last transform pass (per point pair they would be dependent loads after
@@ -1238,12 +1327,14 @@ struct ConsDisp {
     bool has_H;
   };
   __device__ Ctx ctx() const { return Ctx{in.sc->offset, in.sc->H, in.sc->has_H != 0}; }
-  // wo: stored value (gap on comp, displacement elsewhere); v: u_new;
-  // w: the new stored value
-  __device__ uint8_t one(const Ctx& cx, bool comp, uint8_t cmv, double wo, double fv, double hv, T v, T& w,
+  // wo: stored value (gap on comp, displacement elsewhere); v: transform
+  // output; wn: the new stored value
+  __device__ uint8_t one(const Ctx& cx, bool comp, uint8_t cmv, double wo, double fv, double hv, TT v, double& wn,
                          double* acc) const {
-    w = comp ? (T)((double)v - hv) : v;
-    const double wn = (double)w;
+    if (INC)
+      wn = wo + (double)v;
+    else
+      wn = comp ? (double)v - hv : (double)v;
     acc[0] = red_apply(RMAX, acc[0], fabs(wn - wo));
     if (comp) {
       const double pen = (cmv & kCur) ? (cx.offset - wn) : 0.0;
@@ -1253,12 +1344,12 @@ struct ConsDisp {
     const bool cn = phase_a_point(cx.has_H, cx.H, cx.offset, fv, wn, comp, acc + 2);
     return (uint8_t)((cmv & (kCur | kGneg)) | (cn ? kNext : 0));
   }
-  __device__ void take(const Geom& g, int row, int col, const Rec& r, T a, T b, double* acc, const Ctx& cx) const {
-    T w0, w1;
-    const uint8_t m0 = one(cx, r.h.c0, r.cm.a, (double)r.u.a, (double)r.f.a, r.h.h0, a, w0, acc);
-    const uint8_t m1 = one(cx, r.h.c1, r.cm.b, (double)r.u.b, (double)r.f.b, r.h.h1, b, w1, acc);
+  __device__ void take(const Geom& g, int row, int col, const Rec& r, TT a, TT b, double* acc, const Ctx& cx) const {
+    double w0, w1;
+    const uint8_t m0 = one(cx, r.h.c0, r.cm.a, r.w.a, r.f.a, r.h.h0, a, w0, acc);
+    const uint8_t m1 = one(cx, r.h.c1, r.cm.b, r.w.b, r.f.b, r.h.h1, b, w1, acc);
     const size_t p = (size_t)row * g.My + col;
-    st2(in.u + p, w0, w1);
+    st2(in.w + p, w0, w1);
     stm(in.cm + p, m0, m1);
   }
   __device__ void finalize(double* a) const {
@@ -1309,14 +1400,17 @@ struct ConsDisp {
 };
 
 // End of a solve: the displacements u = gap + h back on the computational
-// points (undoing ElemPhaseA's substitution) and the elastic energy
-// -sum(f u)/2 over the topography block (:471, :505-506).
+// points (undoing ElemPhaseA's substitution; FP32 plans: everywhere, from the
+// FP64 gap, and the forces from the FP64 forces) and the elastic energy
+// -sum(f u)/2 over the topography block (:471, :505-506), from the FP64
+// values.
 template <typename T>
 struct ElemFinish {
+  using F = T;
   static constexpr int NS = 1;
   __device__ static constexpr int op(int) { return RSUM; }
   struct Rec {
-    P2<T> f, w;
+    P2<double> f, w;
     HPair h;
   };
   const Inst<T>& in;
@@ -1324,23 +1418,23 @@ struct ElemFinish {
   __device__ bool skip() const { return false; }
   __device__ Rec fetch(const Geom& g, int row, int col) const {
     const size_t p = (size_t)row * g.My + col;
-    return Rec{ld2(in.f + p), ld2(in.u + p), ldh(g, in.h, in.surf, row, col)};
+    return Rec{ld2(in.fm + p), ld2(in.w + p), ldh(g, in.h, in.surf, row, col)};
   }
   __device__ void emit(const Geom& g, int row, int col, const Rec& r, double* acc) const {
-    const T u0 = r.h.c0 ? (T)((double)r.w.a + r.h.h0) : r.w.a;
-    const T u1 = r.h.c1 ? (T)((double)r.w.b + r.h.h1) : r.w.b;
-    if (r.h.c0 || r.h.c1) st2(in.u + (size_t)row * g.My + col, u0, u1);
-    if (row < g.nx && col < g.ny) acc[0] += (double)r.f.a * (double)u0;
-    if (row < g.nx && col + 1 < g.ny) acc[0] += (double)r.f.b * (double)u1;
+    const double u0 = r.h.c0 ? r.w.a + r.h.h0 : r.w.a;
+    const double u1 = r.h.c1 ? r.w.b + r.h.h1 : r.w.b;
+    const size_t p = (size_t)row * g.My + col;
+    if (std::is_same<T, double>::value) {
+      if (r.h.c0 || r.h.c1) st2(in.u + p, (T)u0, (T)u1);
+    } else {
+      st2(in.u + p, (T)u0, (T)u1);
+      st2(in.f + p, (T)r.f.a, (T)r.f.b);
+    }
+    if (row < g.nx && col < g.ny) acc[0] += r.f.a * u0;
+    if (row < g.nx && col + 1 < g.ny) acc[0] += r.f.b * u1;
   }
   __device__ void finalize(double* a) const { in.sc->fun = -a[0] / 2; }
 };
-
-// FP32 sessions keep their own FP32 copy of the heights.
-__global__ void k_heights_to_f32(const double* __restrict__ h, float* __restrict__ out, size_t n) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    out[i] = (float)h[i];
-}
 
 // ---------------------------------------------------------------------------
 // Green's function set-up kernels
@@ -1450,15 +1544,15 @@ bool make_line_plan(int L, int max_tpl, int emax, LinePlan& best) {
         pick = R;
         break;
       }
-    if (!pick || np >= 14) return false;
-    lp.radix[np++] = pick;
+    if (!pick || np >= 16) return false;
+    lp.set_radix(np++, pick);
     rem /= pick;
   }
   lp.npass = np;
-  lp.rlast = lp.radix[np - 1];
+  lp.rlast = lp.radix(np - 1);
   int need = 1;  // smallest tpl meeting the register budget
   for (int s = 0; s < np; ++s) {
-    const int R = lp.radix[s];
+    const int R = lp.radix(s);
     const int nbmax = emax / R;
     if (nbmax < 1) return false;
     need = std::max(need, (L / R + nbmax - 1) / nbmax);
@@ -1467,7 +1561,7 @@ bool make_line_plan(int L, int max_tpl, int emax, LinePlan& best) {
   int tpl = need;
   for (int t = need; t <= max_tpl && t <= L; ++t) {
     bool balanced = true;
-    for (int s = 0; s < np; ++s) balanced = balanced && (L / lp.radix[s]) % t == 0;
+    for (int s = 0; s < np; ++s) balanced = balanced && (L / lp.radix(s)) % t == 0;
     if (balanced) {
       tpl = t;
       break;
@@ -1476,7 +1570,7 @@ bool make_line_plan(int L, int max_tpl, int emax, LinePlan& best) {
   lp.tpl = tpl;
   int E = 0;
   for (int s = 0; s < np; ++s) {
-    const int R = lp.radix[s];
+    const int R = lp.radix(s);
     E = std::max(E, ((L / R + tpl - 1) / tpl) * R);
   }
   lp.E = E;
@@ -1491,8 +1585,9 @@ struct TwTables {
   cplx<T>* post = nullptr;  // N2/2+1 entries: exp(-2 pi i k / My)
 };
 
+// tw[k] = exp(-2 pi i k / denom), k < n, rounded from long double.
 template <typename T>
-std::vector<cplx<T>> host_twiddles(int L, int n, int denom) {
+std::vector<cplx<T>> host_twiddles(int n, int denom) {
   std::vector<cplx<T>> v(n);
   const long double twopi = 6.283185307179586476925286766559005768L;
   for (int k = 0; k < n; ++k) {
@@ -1500,7 +1595,6 @@ std::vector<cplx<T>> host_twiddles(int L, int n, int denom) {
     v[k].x = (T)cosl(ang);
     v[k].y = (T)(-sinl(ang));
   }
-  (void)L;
   return v;
 }
 
@@ -1526,6 +1620,12 @@ struct cm_plan {
   FftExec ex;            // executor in plan precision
   void* G = nullptr;     // (N2+1) x Mx, plan precision, scaled for evaluate_disp
   void* K = nullptr;     // (N2+1) x Mx, plan precision, scaled for evaluate_force
+  // FP64 executor and kernels: FP32 plans use them for the set-up transforms
+  // and the periodic FP64 displacement refresh of a solve; FP64 plans alias
+  // ex, G and K
+  FftExec ex64;
+  void* G64 = nullptr;
+  void* K64 = nullptr;
   std::vector<double> G_host;  // FP64 copy (scaled) for inspection
   void* S = nullptr;     // spectrum work buffer for plain operator calls
   size_t bytes_owned = 0;
@@ -1594,8 +1694,8 @@ bool choose_shape(FftExec& ex, const Shape& sh) {
   ex.emax_row = emax_bucket(ex.row.E);
   ex.emax_col = emax_bucket(ex.col.E);
   ex.col_rev = ex.col;
-  for (int q = 0; q < ex.col.npass; ++q) ex.col_rev.radix[q] = ex.col.radix[ex.col.npass - 1 - q];
-  ex.col_rev.rlast = ex.col_rev.radix[ex.col.npass - 1];
+  for (int q = 0; q < ex.col.npass; ++q) ex.col_rev.set_radix(q, ex.col.radix(ex.col.npass - 1 - q));
+  ex.col_rev.rlast = ex.col_rev.radix(ex.col.npass - 1);
   return true;
 }
 
@@ -1612,9 +1712,9 @@ int exec_setup(cm_ctx* ctx, FftExec& ex, int nx, int ny, int Mx, int My, size_t&
     return fail(ctx, CM_E_UNSUPPORTED,
                 "grid " + std::to_string(Mx) + "x" + std::to_string(My) +
                     " not supported: Mx and My/2 must factor into 2, 3, 5 and a column must fit in LDS");
-  auto twr = host_twiddles<T>(g.N2, g.N2, g.N2);
-  auto twc = host_twiddles<T>(Mx, Mx, Mx);
-  auto twp = host_twiddles<T>(My, g.N2 / 2 + 1, My);
+  auto twr = host_twiddles<T>(g.N2, g.N2);
+  auto twc = host_twiddles<T>(Mx, Mx);
+  auto twp = host_twiddles<T>(g.N2 / 2 + 1, My);
   int rc;
   if ((rc = dev_alloc(ctx, &ex.tw_row, twr.size() * cs))) return rc;
   if ((rc = dev_alloc(ctx, &ex.tw_col, twc.size() * cs))) return rc;
@@ -1626,64 +1726,102 @@ int exec_setup(cm_ctx* ctx, FftExec& ex, int nx, int ny, int Mx, int My, size_t&
   return 0;
 }
 
-// Allow more than 64 KB of dynamic LDS for a kernel (once per kernel).
-void allow_big_lds(const void* kern, size_t lds) {
-  if (lds <= 65536) return;
+// Allow more than 64 KB of dynamic LDS for a kernel.  The limit set for each
+// kernel is remembered and raised again whenever a launch needs more (a shape
+// sweep asks for growing sizes); returns false if the runtime refuses.
+bool allow_big_lds(const void* kern, size_t lds) {
+  if (lds <= 65536) return true;
   static std::mutex mu;
-  static std::vector<const void*> done;
+  static std::vector<std::pair<const void*, size_t>> set;
   std::lock_guard<std::mutex> lk(mu);
-  if (std::find(done.begin(), done.end(), kern) != done.end()) return;
-  (void)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max<size_t>(lds, 65536));
-  done.push_back(kern);
+  for (auto& e : set)
+    if (e.first == kern) {
+      if (e.second >= lds) return true;
+      if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+      }
+      e.second = lds;
+      return true;
+    }
+  if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  set.emplace_back(kern, lds);
+  return true;
 }
 
+// A launch whose dynamic LDS the runtime refused is skipped and remembered
+// here; the entry point that enqueued it reports CM_E_UNSUPPORTED.
+thread_local bool t_lds_refused = false;
+bool lds_ok(const void* kern, size_t lds) {
+  if (allow_big_lds(kern, lds)) return true;
+  t_lds_refused = true;
+  return false;
+}
+int launch_status(cm_ctx* ctx) {
+  if (t_lds_refused) {
+    t_lds_refused = false;
+    return fail(ctx, CM_E_UNSUPPORTED, "dynamic LDS size of a launch shape refused by the runtime");
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(ctx, CM_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return 0;
+}
+
+// Launchers: T is the transform precision of ex; the instance table holds
+// instances with the functor's field precision P::F.
 template <typename T, class P>
-void launch_r2c(const FftExec& ex, InstArr<T> ia, int B, int slot, hipStream_t st) {
+void launch_r2c(const FftExec& ex, InstArr<typename P::F> ia, int B, int slot, hipStream_t st) {
   dim3 grid(ex.g.Mx / ex.g.R, B), block(ex.row.tpl * ex.g.R);
   const cplx<T>* tw = (const cplx<T>*)ex.tw_row;
   const cplx<T>* twp = (const cplx<T>*)ex.tw_post;
   if (ex.emax_row == 8) {
-    allow_big_lds((const void*)k_r2c<T, 8, P>, ex.lds_row);
-    hipLaunchKernelGGL((k_r2c<T, 8, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
+    if (lds_ok((const void*)k_r2c<T, 8, P>, ex.lds_row))
+      hipLaunchKernelGGL((k_r2c<T, 8, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
   } else if (ex.emax_row == 16) {
-    allow_big_lds((const void*)k_r2c<T, 16, P>, ex.lds_row);
-    hipLaunchKernelGGL((k_r2c<T, 16, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
+    if (lds_ok((const void*)k_r2c<T, 16, P>, ex.lds_row))
+      hipLaunchKernelGGL((k_r2c<T, 16, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
   } else {
-    allow_big_lds((const void*)k_r2c<T, 32, P>, ex.lds_row);
-    hipLaunchKernelGGL((k_r2c<T, 32, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
+    if (lds_ok((const void*)k_r2c<T, 32, P>, ex.lds_row))
+      hipLaunchKernelGGL((k_r2c<T, 32, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
   }
 }
 
 template <typename T, class P>
-void launch_c2r(const FftExec& ex, InstArr<T> ia, int B, int slot, hipStream_t st) {
+void launch_c2r(const FftExec& ex, InstArr<typename P::F> ia, int B, int slot, hipStream_t st) {
   dim3 grid(ex.g.Mx / ex.g.R, B), block(ex.row.tpl * ex.g.R);
   const cplx<T>* tw = (const cplx<T>*)ex.tw_row;
   const cplx<T>* twp = (const cplx<T>*)ex.tw_post;
   if (ex.emax_row == 8) {
-    allow_big_lds((const void*)k_c2r<T, 8, P>, ex.lds_row);
-    hipLaunchKernelGGL((k_c2r<T, 8, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
+    if (lds_ok((const void*)k_c2r<T, 8, P>, ex.lds_row))
+      hipLaunchKernelGGL((k_c2r<T, 8, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
   } else if (ex.emax_row == 16) {
-    allow_big_lds((const void*)k_c2r<T, 16, P>, ex.lds_row);
-    hipLaunchKernelGGL((k_c2r<T, 16, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
+    if (lds_ok((const void*)k_c2r<T, 16, P>, ex.lds_row))
+      hipLaunchKernelGGL((k_c2r<T, 16, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
   } else {
-    allow_big_lds((const void*)k_c2r<T, 32, P>, ex.lds_row);
-    hipLaunchKernelGGL((k_c2r<T, 32, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
+    if (lds_ok((const void*)k_c2r<T, 32, P>, ex.lds_row))
+      hipLaunchKernelGGL((k_c2r<T, 32, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
   }
 }
 
-template <typename T, int MODE>
-void launch_col(const FftExec& ex, InstArr<T> ia, int B, int useK, int skip, hipStream_t st) {
+template <typename T, int MODE, typename TF = T>
+void launch_col(const FftExec& ex, InstArr<TF> ia, int B, int useK, int skip, hipStream_t st) {
   dim3 grid(ex.g.N2 / ex.g.Cb, B), block(ex.col.tpl * ex.g.Cb);
   const cplx<T>* tw = (const cplx<T>*)ex.tw_col;
   if (ex.emax_col == 8) {
-    allow_big_lds((const void*)k_col<T, 8, MODE>, ex.lds_col);
-    hipLaunchKernelGGL((k_col<T, 8, MODE>), grid, block, ex.lds_col, st, ex.g, ex.col, ex.col_rev, tw, ia, useK, skip);
+    if (lds_ok((const void*)k_col<T, 8, MODE, TF>, ex.lds_col))
+      hipLaunchKernelGGL((k_col<T, 8, MODE, TF>), grid, block, ex.lds_col, st, ex.g, ex.col, ex.col_rev, tw, ia, useK,
+                         skip);
   } else if (ex.emax_col == 16) {
-    allow_big_lds((const void*)k_col<T, 16, MODE>, ex.lds_col);
-    hipLaunchKernelGGL((k_col<T, 16, MODE>), grid, block, ex.lds_col, st, ex.g, ex.col, ex.col_rev, tw, ia, useK, skip);
+    if (lds_ok((const void*)k_col<T, 16, MODE, TF>, ex.lds_col))
+      hipLaunchKernelGGL((k_col<T, 16, MODE, TF>), grid, block, ex.lds_col, st, ex.g, ex.col, ex.col_rev, tw, ia,
+                         useK, skip);
   } else {
-    allow_big_lds((const void*)k_col<T, 32, MODE>, ex.lds_col);
-    hipLaunchKernelGGL((k_col<T, 32, MODE>), grid, block, ex.lds_col, st, ex.g, ex.col, ex.col_rev, tw, ia, useK, skip);
+    if (lds_ok((const void*)k_col<T, 32, MODE, TF>, ex.lds_col))
+      hipLaunchKernelGGL((k_col<T, 32, MODE, TF>), grid, block, ex.lds_col, st, ex.g, ex.col, ex.col_rev, tw, ia,
+                         useK, skip);
   }
 }
 
@@ -1702,6 +1840,13 @@ int plan_free(cm_plan* p) {
   dev_free(ctx, p->G, 0);
   dev_free(ctx, p->K, 0);
   dev_free(ctx, p->S, 0);
+  if (p->prec != CM_FP64) {
+    dev_free(ctx, p->ex64.tw_row, 0);
+    dev_free(ctx, p->ex64.tw_col, 0);
+    dev_free(ctx, p->ex64.tw_post, 0);
+    dev_free(ctx, p->G64, 0);
+    dev_free(ctx, p->K64, 0);
+  }
   ctx->used -= std::min(ctx->used, p->bytes_owned);
   delete p;
   return 0;
@@ -1774,6 +1919,7 @@ int free_green_device(cm_plan* P, std::vector<double>& G) {
   ia.one.S = S;
   launch_r2c<double, ProdCopy<double>>(ex, ia, 1, 0, st);
   launch_col<double, 1>(ex, ia, 1, 0, 0, st);
+  if ((rc = launch_status(ctx))) return rc;
   hipLaunchKernelGGL(k_unpack_green, dim3(1024), dim3(256), 0, st, ex.g, S, Gd);
   G.resize(gn);
   HIPCHK(ctx, hipMemcpyAsync(G.data(), Gd, gn * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1788,8 +1934,9 @@ int free_green_device(cm_plan* P, std::vector<double>& G) {
   return 0;
 }
 
+// Scaled evaluate_disp / evaluate_force kernels in precision T into Gd, Kd.
 template <typename T>
-int upload_kernels(cm_plan* P, const std::vector<double>& Graw) {
+int upload_kernels(cm_plan* P, const std::vector<double>& Graw, void*& Gd, void*& Kd) {
   cm_ctx* ctx = P->ctx;
   const size_t n = Graw.size();
   const double M = (double)P->Mx * P->My;
@@ -1804,11 +1951,11 @@ int upload_kernels(cm_plan* P, const std::vector<double>& Graw) {
     Ks[i] = (T)(kv * (-P->area_per_pt / M));
   }
   int rc;
-  if ((rc = dev_alloc(ctx, &P->G, n * sizeof(T)))) return rc;
-  if ((rc = dev_alloc(ctx, &P->K, n * sizeof(T)))) return rc;
+  if ((rc = dev_alloc(ctx, &Gd, n * sizeof(T)))) return rc;
+  if ((rc = dev_alloc(ctx, &Kd, n * sizeof(T)))) return rc;
   P->bytes_owned += 2 * n * sizeof(T);
-  HIPCHK(ctx, hipMemcpy(P->G, Gs.data(), n * sizeof(T), hipMemcpyHostToDevice));
-  HIPCHK(ctx, hipMemcpy(P->K, Ks.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(Gd, Gs.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(Kd, Ks.data(), n * sizeof(T), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -1822,7 +1969,15 @@ int plan_build(cm_plan* P) {
     periodic_green_host(P->d, P->Mx, P->My, Graw);
   else if ((rc = free_green_device(P, Graw)))
     return rc;
-  if ((rc = upload_kernels<T>(P, Graw))) return rc;
+  if ((rc = upload_kernels<T>(P, Graw, P->G, P->K))) return rc;
+  if (std::is_same<T, double>::value) {
+    P->ex64 = P->ex;
+    P->G64 = P->G;
+    P->K64 = P->K;
+  } else {
+    if ((rc = exec_setup<double>(ctx, P->ex64, P->d.nx, P->d.ny, P->Mx, P->My, P->bytes_owned))) return rc;
+    if ((rc = upload_kernels<double>(P, Graw, P->G64, P->K64))) return rc;
+  }
   const size_t Sb = (size_t)P->Mx * (P->My / 2) * sizeof(cplx<T>);
   if ((rc = dev_alloc(ctx, &P->S, Sb))) return rc;
   P->bytes_owned += Sb;
@@ -1880,11 +2035,16 @@ int autotune(cm_plan* P, int reps, std::string& desc) {
           FftExec trial = P->ex;
           if (!choose_shape<T>(trial, Shape{Cb, R, er, ec})) continue;
           evaluate_ex<T>(P, trial, a, b, 0, st);  // warm-up (first launch of a kernel)
+          if (t_lds_refused) {  // the runtime refused this shape's LDS: not a candidate
+            t_lds_refused = false;
+            continue;
+          }
+          if ((rc = launch_status(ctx))) return rc;
           HIPCHK(ctx, hipEventRecord(e0, st));
           for (int k = 0; k < reps; ++k) evaluate_ex<T>(P, trial, a, b, 0, st);
           HIPCHK(ctx, hipEventRecord(e1, st));
           HIPCHK(ctx, hipEventSynchronize(e1));
-          if (hipGetLastError() != hipSuccess) continue;  // shape rejected by the runtime
+          if ((rc = launch_status(ctx))) return rc;
           float ms = 0;
           HIPCHK(ctx, hipEventElapsedTime(&ms, e0, e1));
           const float us = ms * 1000.f / reps;
@@ -1922,16 +2082,15 @@ int evaluate(cm_plan* P, const void* in, void* out, int useK, void* stream) {
   launch_r2c<T, ProdCopy<T>>(P->ex, ia, 1, 0, st);
   launch_col<T, 0>(P->ex, ia, 1, useK, 0, st);
   launch_c2r<T, ConsCopy<T>>(P->ex, ia, 1, 0, st);
-  HIPCHK(P->ctx, hipGetLastError());
-  return 0;
+  return launch_status(P->ctx);
 }
 
 // ---------------------------------------------------------------------------
 // CCG driver
 
 struct WsLayout {
-  size_t t, S, sc, part, ctr, hc, inst, inst0, total;
-  size_t st_t, st_S, st_sc, st_part, st_ctr, st_hc;  // per-instance strides
+  size_t t, S, sc, part, ctr, w, fm, S64, inst, inst0, total;
+  size_t st_t, st_S, st_sc, st_part, st_ctr, st_w, st_fm, st_S64;  // per-instance strides
 };
 
 size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -1940,9 +2099,10 @@ WsLayout ws_layout(const cm_plan* P, int B) {
   const size_t w = P->prec == CM_FP64 ? 8 : 4;
   const size_t M = (size_t)P->Mx * P->My;
   // partials: up to 8 slots per block of any reducing launch (element-wise,
-  // row passes, column pass)
+  // row passes, column pass) of either executor
   const size_t nblk_max = std::max<size_t>({(size_t)P->nblk_elem, (size_t)(P->Mx / P->ex.g.R),
-                                            (size_t)(P->ex.g.N2 / P->ex.g.Cb)});
+                                            (size_t)(P->ex.g.N2 / P->ex.g.Cb), (size_t)(P->Mx / P->ex64.g.R),
+                                            (size_t)(P->ex64.g.N2 / P->ex64.g.Cb)});
   const size_t nparts = nblk_max * 8;
   WsLayout L{};
   L.st_t = al256(M * w);
@@ -1950,7 +2110,10 @@ WsLayout ws_layout(const cm_plan* P, int B) {
   L.st_sc = al256(sizeof(Scal));
   L.st_part = al256(nparts * sizeof(double));
   L.st_ctr = al256(8 * kCtrSlot * sizeof(unsigned));
-  L.st_hc = w == 4 ? al256((size_t)P->d.nx * P->d.ny * 4) : 0;  // FP32 copy of the heights
+  // FP32 plans: FP64 gap and forces, FP64 spectrum of the refresh transforms
+  L.st_w = w == 4 ? al256(M * 8) : 0;
+  L.st_fm = w == 4 ? al256(M * 8) : 0;
+  L.st_S64 = w == 4 ? al256((M / 2) * 16) : 0;
   const size_t inst_bytes = al256(std::max(sizeof(Inst<double>), sizeof(Inst<float>)) * B);
   size_t o = 0;
   L.t = o; o += L.st_t * B;
@@ -1958,7 +2121,9 @@ WsLayout ws_layout(const cm_plan* P, int B) {
   L.sc = o; o += L.st_sc * B;
   L.part = o; o += L.st_part * B;
   L.ctr = o; o += L.st_ctr * B;
-  L.hc = o; o += L.st_hc * B;
+  L.w = o; o += L.st_w * B;
+  L.fm = o; o += L.st_fm * B;
+  L.S64 = o; o += L.st_S64 * B;
   L.inst = o; o += inst_bytes;
   L.inst0 = o; o += inst_bytes;
   L.total = o;
@@ -1994,6 +2159,8 @@ void fill_stats(const Scal& s, cm_ccg_stats* o) {
 struct cm_session {
   cm_plan* P = nullptr;
   FftExec ex;  // the plan's launch shapes when the session began (the workspace is sized for them)
+  FftExec ex64;  // FP64 executor (FP32 plans: set-up and refresh transforms; FP64 plans: == ex)
+  int refresh_every = 0;  // FP32 plans: every k-th iteration's displacement update is an FP64 refresh
   int B = 0;
   hipStream_t st = nullptr;
   void* ws = nullptr;
@@ -2007,14 +2174,41 @@ struct cm_session {
   Scal* host = nullptr;  // pinned copy of the B scalar records
   void* hin = nullptr;   // pinned instance tables (main, set-up)
   void* hin0 = nullptr;
-  std::vector<std::pair<int, hipGraphExec_t>> graphs;
+  std::vector<std::pair<long long, hipGraphExec_t>> graphs;  // key: chunk length, refresh phase
   std::vector<hipGraph_t> graph_defs;
 };
 
 namespace {
 
+// Whether iteration number k (0-based, counted over the session) updates the
+// displacements with an FP64 refresh: always for FP64 plans; for FP32 plans
+// every refresh_every-th iteration, starting with the first.
+bool refresh_iteration(const cm_session* S, long long k) {
+  return S->P->prec == CM_FP64 || S->refresh_every <= 1 || k % S->refresh_every == 0;
+}
+
+// The second convolution of an iteration and the displacement pass: FP64 on
+// FP64 plans; FP32 plans take the FP64 transforms of the new forces on
+// refresh iterations and the FP32 transforms of the force change otherwise.
 template <typename T>
-void enqueue_iteration(cm_session* S) {
+void enqueue_displacement_update(cm_session* S, InstArr<T> ia, bool refresh) {
+  const int B = S->B;
+  hipStream_t st = S->st;
+  if (refresh) {
+    const FftExec& e64 = S->ex64;
+    launch_r2c<double, ProdForce<T, double, false>>(e64, ia, B, 6, st);
+    launch_col<double, 0, T>(e64, ia, B, 0, 1, st);
+    launch_c2r<double, ConsDisp<T, double, false>>(e64, ia, B, 7, st);
+  } else if constexpr (!std::is_same<T, double>::value) {
+    const FftExec& ex = S->ex;
+    launch_r2c<T, ProdForce<T, T, true>>(ex, ia, B, 6, st);
+    launch_col<T, 0>(ex, ia, B, 0, 1, st);
+    launch_c2r<T, ConsDisp<T, T, true>>(ex, ia, B, 7, st);
+  }
+}
+
+template <typename T>
+void enqueue_iteration(cm_session* S, bool refresh) {
   const FftExec& ex = S->ex;
   InstArr<T> ia{};
   ia.arr = (const Inst<T>*)((char*)S->ws + S->L.inst);
@@ -2026,10 +2220,10 @@ void enqueue_iteration(cm_session* S) {
   launch_r2c<T, ProdDir<T>>(ex, ia, B, 3, st);
   launch_col<T, 3>(ex, ia, B, 0, 2, st);  // tau from the spectrum: no inverse pass
   launch_elem<T, ElemPhaseB<T>>(ex, nbe, ia, B, 5, st);
-  launch_r2c<T, ProdForce<T>>(ex, ia, B, 6, st);
-  launch_col<T, 0>(ex, ia, B, 0, 1, st);
-  launch_c2r<T, ConsDisp<T>>(ex, ia, B, 7, st);
+  enqueue_displacement_update<T>(S, ia, refresh);
 }
+
+void session_free(cm_session* S);
 
 template <typename T>
 int session_begin(cm_plan* P, int B, const cm_ccg_params* prm, const double* const* hs,
@@ -2042,6 +2236,8 @@ int session_begin(cm_plan* P, int B, const cm_ccg_params* prm, const double* con
   cm_session* S = new cm_session();
   S->P = P;
   S->ex = P->ex;
+  S->ex64 = std::is_same<T, double>::value ? P->ex : P->ex64;
+  S->refresh_every = prm[0].refresh_every > 0 ? prm[0].refresh_every : kRefreshEvery;
   S->B = B;
   S->st = stream ? (hipStream_t)stream : ctx->stream;
   S->L = ws_layout(P, B);
@@ -2070,22 +2266,19 @@ int session_begin(cm_plan* P, int B, const cm_ccg_params* prm, const double* con
     Inst<T>& I = hin[b];
     std::memset(&I, 0, sizeof(I));
     I.u = (T*)us[b];
+    I.w = std::is_same<T, double>::value ? (double*)us[b] : (double*)(base + L.w + b * L.st_w);
     I.f = (T*)fs[b];
+    I.fm = std::is_same<T, double>::value ? (double*)fs[b] : (double*)(base + L.fm + b * L.st_fm);
     I.t = (T*)(base + L.t + b * L.st_t);
     I.cm = cms[b];
-    if (std::is_same<T, double>::value) {
-      I.h = (const T*)hs[b];
-    } else {
-      float* hc = (float*)(base + L.hc + b * L.st_hc);
-      const size_t nh = (size_t)P->d.nx * P->d.ny;
-      hipLaunchKernelGGL(k_heights_to_f32, dim3((unsigned)std::min<size_t>(1024, (nh + 255) / 256)), dim3(256), 0,
-                         S->st, hs[b], hc, nh);
-      I.h = (const T*)hc;
-    }
+    I.h = hs[b];
     I.surf = surfs ? surfs[b] : nullptr;
     I.S = (cplx<T>*)(base + L.S + b * L.st_S);
     I.G = (const T*)P->G;
     I.K = (const T*)P->K;
+    I.S64 = std::is_same<T, double>::value ? (cplx<double>*)I.S : (cplx<double>*)(base + L.S64 + b * L.st_S64);
+    I.G64 = (const double*)P->G64;
+    I.K64 = (const double*)P->K64;
     I.sc = (Scal*)(base + L.sc + b * L.st_sc);
     I.part = (double*)(base + L.part + b * L.st_part);
     I.ctr = (unsigned*)(base + L.ctr + b * L.st_ctr);
@@ -2101,10 +2294,7 @@ int session_begin(cm_plan* P, int B, const cm_ccg_params* prm, const double* con
     Scal& s = hsc[b];
     std::memset(&s, 0, sizeof(Scal));
     s.F = q.external_force;
-    // FP32 plans store forces in FP32: with the cap rounded to FP32, a force
-    // clipped to -H is exactly -H again when read back, so the flowing test
-    // f <= -H stays true for it (an unrounded cap can flip it every pass)
-    s.H = std::is_same<T, float>::value ? (double)(float)q.hardness : q.hardness;
+    s.H = q.hardness;
     s.pentol = q.pentol;
     s.forcetol = q.forcetol;
     s.thermotol = q.thermotol;
@@ -2145,21 +2335,25 @@ int session_begin(cm_plan* P, int B, const cm_ccg_params* prm, const double* con
   InstArr<T> ia0{};
   ia0.arr = dinst0;
   const FftExec& ex = S->ex;
+  const FftExec& e64 = S->ex64;
   const int nbe = P->nblk_elem;
-  // ---- set-up (:179-219)
+  // ---- set-up (:179-219); the initial transform is FP64 in both precisions
   launch_elem<T, ElemInit<T>>(ex, nbe, ia, B, 0, st);
   if (prm[0].init_mode == 0) {
-    launch_r2c<T, ProdCopy<T>>(ex, ia0, B, 1, st);  // f = evaluate_force(u), f[pad] = 0
-    launch_col<T, 0>(ex, ia0, B, 1, 0, st);
-    launch_c2r<T, ConsInitForce<T>>(ex, ia0, B, 1, st);
+    launch_r2c<double, ProdCopy<T, double>>(e64, ia0, B, 1, st);  // f = evaluate_force(u), f[pad] = 0
+    launch_col<double, 0, T>(e64, ia0, B, 1, 0, st);
+    launch_c2r<double, ConsInitForce<T, double>>(e64, ia0, B, 1, st);
   } else {
-    launch_r2c<T, ProdCopy<T>>(ex, ia0, B, 1, st);  // u = evaluate_disp(f)
-    launch_col<T, 0>(ex, ia0, B, 0, 0, st);
-    launch_c2r<T, ConsCopy<T>>(ex, ia0, B, 1, st);
+    launch_r2c<double, ProdCopy<T, double>>(e64, ia0, B, 1, st);  // u = evaluate_disp(f)
+    launch_col<double, 0, T>(e64, ia0, B, 0, 0, st);
+    launch_c2r<double, ConsCopy<T, double>>(e64, ia0, B, 1, st);
     launch_elem<T, ElemZeroPad<T>>(ex, nbe, ia, B, 1, st);
   }
   launch_elem<T, ElemPhaseA<T>>(ex, nbe, ia, B, 2, st);  // start of iteration 1
-  HIPCHK(ctx, hipGetLastError());
+  if ((rc = launch_status(ctx))) {
+    session_free(S);
+    return rc;
+  }
   *out = S;
   return 0;
 }
@@ -2171,14 +2365,18 @@ int session_step(cm_session* S, int n) {
   cm_ctx* ctx = S->P->ctx;
   if (n <= 0) return 0;
   if (S->use_graph && n >= 8) {
+    // a captured chunk is valid for its length and for where it starts in
+    // the refresh cycle of an FP32 plan
+    const long long phase = S->refresh_every > 1 ? S->issued % S->refresh_every : 0;
+    const long long key = (long long)n * 4096 + phase;
     hipGraphExec_t gx = nullptr;
     for (auto& g : S->graphs)
-      if (g.first == n) gx = g.second;
+      if (g.first == key) gx = g.second;
     if (!gx) {
       hipGraph_t gdef = nullptr;
       hipError_t e = hipStreamBeginCapture(S->st, hipStreamCaptureModeThreadLocal);
       if (e == hipSuccess) {
-        for (int k = 0; k < n; ++k) enqueue_iteration<T>(S);
+        for (int k = 0; k < n; ++k) enqueue_iteration<T>(S, refresh_iteration(S, S->issued + k));
         e = hipStreamEndCapture(S->st, &gdef);
       }
       if (e == hipSuccess) e = hipGraphInstantiate(&gx, gdef, nullptr, nullptr, 0);
@@ -2188,7 +2386,7 @@ int session_step(cm_session* S, int n) {
         S->use_graph = 0;  // capture unavailable on this stream: launch directly
         gx = nullptr;
       } else {
-        S->graphs.emplace_back(n, gx);
+        S->graphs.emplace_back(key, gx);
         S->graph_defs.push_back(gdef);
       }
     }
@@ -2198,10 +2396,9 @@ int session_step(cm_session* S, int n) {
       return 0;
     }
   }
-  for (int k = 0; k < n; ++k) enqueue_iteration<T>(S);
+  for (int k = 0; k < n; ++k) enqueue_iteration<T>(S, refresh_iteration(S, S->issued + k));
   S->issued += n;
-  HIPCHK(ctx, hipGetLastError());
-  return 0;
+  return launch_status(ctx);
 }
 
 template <typename T>
@@ -2241,12 +2438,22 @@ int session_profile(cm_session* S, int n, float* us) {
     // (tau comes from the spectrum in the column pass: no c2r pass)
     HIPCHK(ctx, hipEventRecord(ev[k++], st));
     launch_elem<T, ElemPhaseB<T>>(ex, nbe, ia, B, 5, st);
+    // the displacement update as between refreshes: FP64 transforms of the
+    // new forces (FP64 plans), FP32 transforms of the force change (FP32)
     HIPCHK(ctx, hipEventRecord(ev[k++], st));
-    launch_r2c<T, ProdForce<T>>(ex, ia, B, 6, st);
-    HIPCHK(ctx, hipEventRecord(ev[k++], st));
-    launch_col<T, 0>(ex, ia, B, 0, 1, st);
-    HIPCHK(ctx, hipEventRecord(ev[k++], st));
-    launch_c2r<T, ConsDisp<T>>(ex, ia, B, 7, st);
+    if constexpr (std::is_same<T, double>::value) {
+      launch_r2c<T, ProdForce<T, T, false>>(ex, ia, B, 6, st);
+      HIPCHK(ctx, hipEventRecord(ev[k++], st));
+      launch_col<T, 0>(ex, ia, B, 0, 1, st);
+      HIPCHK(ctx, hipEventRecord(ev[k++], st));
+      launch_c2r<T, ConsDisp<T, T, false>>(ex, ia, B, 7, st);
+    } else {
+      launch_r2c<T, ProdForce<T, T, true>>(ex, ia, B, 6, st);
+      HIPCHK(ctx, hipEventRecord(ev[k++], st));
+      launch_col<T, 0>(ex, ia, B, 0, 1, st);
+      HIPCHK(ctx, hipEventRecord(ev[k++], st));
+      launch_c2r<T, ConsDisp<T, T, true>>(ex, ia, B, 7, st);
+    }
     HIPCHK(ctx, hipEventRecord(ev[k++], st));
     HIPCHK(ctx, hipEventSynchronize(ev[NK]));
     for (int i = 0; i < NK; ++i) {

@@ -17,7 +17,7 @@ from . import _native
 
 def make_params(hardness=None, external_force=None, offset=None, pentol=1e-8, forcetol=1e-5,
                 thermotol=1e-6, mixfac=0.1, mixdecfac=0.95, minmixsteps=10, maxiter=100000,
-                init_mode=0, iters_per_sync=0, use_graph=True):
+                init_mode=0, iters_per_sync=0, use_graph=True, refresh_every=0):
     """``cm_ccg_params`` from the reference keyword arguments."""
     return _native.CcgParams(
         load_control=1 if external_force is not None else 0,
@@ -28,7 +28,7 @@ def make_params(hardness=None, external_force=None, offset=None, pentol=1e-8, fo
         pentol=float(pentol), forcetol=float(forcetol), thermotol=float(thermotol),
         mixfac=float(mixfac), mixdecfac=float(mixdecfac), minmixsteps=int(minmixsteps),
         maxiter=int(maxiter), init_mode=int(init_mode), iters_per_sync=int(iters_per_sync),
-        use_graph=int(bool(use_graph)))
+        use_graph=int(bool(use_graph)), refresh_every=int(refresh_every))
 
 
 class CcgSession:

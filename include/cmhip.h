@@ -15,8 +15,11 @@
  * forces (not pressures) with compressive force negative, as in the reference.
  *
  * Precision: CM_FP64 keeps every field and transform in double.  CM_FP32
- * keeps fields/spectra in float while all global reductions, the CG scalars
- * and the gap arithmetic run in double.
+ * keeps forces, conjugate directions and the transforms of the iteration in
+ * float; the gap u - h is kept in double, updated by the FP32 transform of
+ * the force change and recomputed with FP64 transforms every refresh_every
+ * iterations; all global reductions, the CG scalars and the convergence
+ * logic run in double.
  */
 #ifndef CMHIP_H
 #define CMHIP_H
@@ -80,6 +83,9 @@ typedef struct cm_ccg_params {
   int init_mode;           /* 0: forces from initial displacements, 1: initial forces given */
   int iters_per_sync;      /* 0 = adaptive; 1 = host sees every iteration */
   int use_graph;           /* 1 = replay chunks of iterations from a captured hipGraph */
+  int refresh_every;       /* FP32 plans: the displacements are recomputed from the forces with FP64
+                              transforms every refresh_every iterations (0 = default 8) and updated
+                              by FP32 transforms of the force change in between */
 } cm_ccg_params;
 
 /* Per-solve statistics; the scalar fields of the reference OptimizeResult

@@ -21,5 +21,5 @@ reference_solutions
 Parity status of each function (what pins it) is listed in the module
 docstrings.  Functions whose behaviour lives in third-party packages absent
 from the reference tree (SurfaceTopography's ``rms_height_from_area``) are
-marked "parity unpinned" where they appear.
+restated from their published definition and pinned by closed forms.
 """

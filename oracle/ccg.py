@@ -20,9 +20,10 @@ trajectory recorded in the reference notebook
 (``examples/plastic_simulation_cone.ipynb`` cell 15) and the bearing-area limit
 at vanishing hardness.
 
-``rms_height_from_area`` (SurfaceTopography, absent) is restated as the
-area-RMS of the heights about their mean: parity unpinned; every test and
-workload passes ``pentol`` explicitly.
+``rms_height_from_area`` (SurfaceTopography >= 1.20, absent from the
+reference tree) is restated as the area RMS of the valid heights about their
+mean, pinned by closed forms in ``tests/test_oracle_pins.py`` (sinusoid
+A/sqrt(2), two-level field, constant, masked pixels excluded).
 """
 
 from math import isnan, sqrt
@@ -45,7 +46,9 @@ class Result(dict):
 
 
 def rms_height_from_area(heights):
-    """Area RMS height about the mean (parity unpinned, see module docstring)."""
+    """Area RMS height about the mean of the valid pixels (SurfaceTopography's
+    ``rms_height_from_area`` on a uniform grid; closed-form pins in the module
+    docstring)."""
     h = np.asarray(np.ma.filled(heights, np.nan), dtype=float)
     h = h[np.isfinite(h)]
     return float(np.sqrt(np.mean((h - h.mean()) ** 2)))

@@ -274,15 +274,25 @@ class FreeHalfSpace(PeriodicHalfSpace):
                 forces.shape, self.nb_subdomain_grid_pts, self.nb_domain_grid_pts))
 
     def check_boundaries(self, force=None, tol=0):
+        """Reference ``FFTElasticHalfSpace.py:1217-1269`` (serial): the four
+        edges of the topography block must carry |force| <= tol; with a
+        masked force, an edge passes if its unmasked values do or if it is
+        masked entirely (:1236-1243)."""
         if force is None:
             force = self.force
         nx, ny = self.nb_grid_pts
-        ok = bool((abs(force[:, 0]) <= tol).all())
-        if ny - 1 > 0:
-            ok &= bool((abs(force[:, ny - 1]) <= tol).all())
-        ok &= bool((abs(force[0, :]) <= tol).all())
-        if nx - 1 > 0:
-            ok &= bool((abs(force[nx - 1, :]) <= tol).all())
+        if np.ma.is_masked(force):
+            def edge(v):
+                return bool((abs(v) <= tol).all() or v.mask.all())
+        else:
+            def edge(v):
+                return bool((abs(v) <= tol).all())
+        ok = edge(force[:, 0])
+        if 0 < ny - 1 < force.shape[1]:
+            ok &= edge(force[:, ny - 1])
+        ok &= edge(force[0, :])
+        if 0 < nx - 1 < force.shape[0]:
+            ok &= edge(force[nx - 1, :])
         if not ok:
             raise self.FreeBoundaryError(
                 "The forces not zero at the boundary of the active domain.")

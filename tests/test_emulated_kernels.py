@@ -49,9 +49,19 @@ def test_emulated_plastic_free_solve_follows_oracle(emu_lib):
     assert r["rc"] == 0 and r["nit"] == r["nit_oracle"] and r["rel"] < 1e-8
 
 
-def test_emulated_fp32_solve(emu_lib):
-    r = run_case(dict(kind="solve", nx=32, ny=32, bc=0, prec=1, seed=3, force=3e-3, pentol=1e-6, forcetol=1e-10))
-    assert r["rc"] == 0 and r["rel"] < 1e-5
+@pytest.mark.parametrize("bc,hardness", [(0, None), (1, 6e-5)])
+def test_emulated_fp32_solve_reaches_oracle_at_tight_tolerance(emu_lib, bc, hardness):
+    # FP32 transforms with FP64 forces and gaps (incremental displacement
+    # updates, FP64 refresh every 8 iterations): converges at pentol 1e-10 in
+    # the oracle's iteration count; the forces come back rounded to FP32
+    case = dict(kind="solve", nx=32 if bc == 0 else 24, ny=32 if bc == 0 else 20, bc=bc, prec=1,
+                seed=3 if bc == 0 else 5, force=3e-3, pentol=1e-10, forcetol=1e-14)
+    if hardness:
+        case["hardness"] = hardness
+    r = run_case(case)
+    assert r["rc"] == 0 and r["success_oracle"]
+    assert abs(r["nit"] - r["nit_oracle"]) <= 1
+    assert r["rel"] < 1e-7
 
 
 @pytest.mark.parametrize("nx,ny,bc", [(48, 40, 0), (24, 30, 1)])

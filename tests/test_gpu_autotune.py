@@ -20,11 +20,20 @@ from oracle.halfspace import FreeHalfSpace, PeriodicHalfSpace
 pytestmark = pytest.mark.gpu
 
 
+# Explicit launch shapes on FP32 plans: a 2048^2 FP32 solve run over
+# non-automatic shapes ended in a GPU memory fault whose cause was not found;
+# the library refuses explicit shapes for FP32 plans (CM_E_UNSUPPORTED), so
+# this case is kept, skipped, until the cause is shown and fixed.
+FP32_SHAPES_SKIP = pytest.mark.skip(reason="explicit FP32 launch shapes are refused by the library after an "
+                                           "unexplained GPU memory fault in a 2048^2 FP32 solve over them")
+
+
 @pytest.mark.parametrize("cls,ocls,nx,ny,prec", [
     (PeriodicFFTElasticHalfSpace, PeriodicHalfSpace, 128, 96, "fp64"),
     (PeriodicFFTElasticHalfSpace, PeriodicHalfSpace, 256, 256, "fp64"),
     (PeriodicFFTElasticHalfSpace, PeriodicHalfSpace, 2048, 64, "fp64"),
     (FreeFFTElasticHalfSpace, FreeHalfSpace, 48, 80, "fp64"),
+    pytest.param(PeriodicFFTElasticHalfSpace, PeriodicHalfSpace, 256, 256, "fp32", marks=FP32_SHAPES_SKIP),
 ])
 def test_all_shapes_agree(gpu, cls, ocls, nx, ny, prec):
     g = cls((nx, ny), 1.7, (1.0, 2.0), precision=prec)

@@ -192,6 +192,31 @@ def test_system_api(gpu):
     assert np.min(gap) > -1e-9
 
 
+@pytest.mark.parametrize("bc,hard", [("periodic", False), ("free", False), ("free", True)])
+def test_fp32_solve_matches_oracle(gpu, bc, hard):
+    """FP32 transforms and conjugate directions, FP64 forces, gaps and
+    reductions (displacements updated by the FP32 transform of the force
+    change, recomputed with FP64 transforms every 8 iterations): at tight
+    tolerances the solve converges like the FP64 oracle and the pressures
+    agree to the FP32 rounding of the returned forces."""
+    n = 256 if bc == "periodic" else 128
+    h = rough(n, 31)
+    gcls, ocls = ((PeriodicFFTElasticHalfSpace, PeriodicHalfSpace) if bc == "periodic"
+                  else (FreeFFTElasticHalfSpace, FreeHalfSpace))
+    kw = dict(external_force=0.01, pentol=1e-11, forcetol=1e-14, thermotol=1e-10, maxiter=20000)
+    if hard:
+        kw["hardness"] = 0.05 / (n * n)
+    rg = ccg_gpu(gcls((n, n), 1.0, (1.0, 1.0), precision="fp32"), h, **kw)
+    rc = ccg_cpu(ocls((n, n), 1.0, (1.0, 1.0)), h, **kw)
+    assert rg.success and rc.success
+    l2 = np.linalg.norm(rg.jac - rc.jac) / np.linalg.norm(rc.jac)
+    print(f"fp32 {bc} hard={hard}: nit {rg.nit} (oracle {rc.nit}), L2 {l2:.2e}, Linf {rel(rg.jac, rc.jac):.2e}")
+    assert l2 < 1e-6 and rel(rg.jac, rc.jac) < 1e-6
+    assert np.sum((rg.jac > 0) != (rc.jac > 0)) <= 2
+    np.testing.assert_allclose(rg.offset, rc.offset, rtol=1e-7)
+    assert abs(rg.nit - rc.nit) <= max(3, rc.nit // 20)
+
+
 @pytest.mark.parametrize("bc", ["periodic", "free"])
 def test_mixed_precision_reaches_fp64_accuracy(gpu, bc):
     # FP32 bulk iterations + FP64 finish: within 1e-6 (L2, relative) of the

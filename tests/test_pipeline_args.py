@@ -1,11 +1,12 @@
-"""Argument checks of the load-sweep driver (CPU: raised before any GPU use),
-reference ``PipelineFunction.py:301-330``."""
+"""Host logic of the load-sweep driver (CPU: no GPU use), reference
+``PipelineFunction.py:138-227`` (automatic offsets) and :301-330 (argument
+checks)."""
 
 import numpy as np
 import pytest
 
 from contactmechanics_amd import Topography
-from contactmechanics_amd.PipelineFunction import _next_offset, contact_mechanics
+from contactmechanics_amd.PipelineFunction import automatic_offset, contact_mechanics
 
 
 def test_exactly_one_sweep_mode():
@@ -16,16 +17,33 @@ def test_exactly_one_sweep_mode():
         contact_mechanics(t, substrate="periodic", nsteps=3, pressures=[1e-3])
     with pytest.raises(ValueError):
         contact_mechanics(t, substrate="periodic", offsets=[0.1], pressures=[1e-3])
+    with pytest.raises(ValueError):
+        contact_mechanics(t, substrate="periodic", nsteps=2, offsets=[0.1])
 
 
-def test_automatic_offsets_follow_the_reference_rule():
-    rng = np.random.default_rng(0)
-    t = Topography(rng.standard_normal((16, 16)), (1.0, 1.0))
-    h = t.heights()
-    assert _next_offset(t, None) == -np.mean(h)
-    hist = (np.array([-np.mean(h)]), None, None, np.array([0.3]), None)
-    assert _next_offset(t, hist) == -np.max(h) + 0.01 * (np.max(h) - np.mean(h))
-    # step >= 2: bisect the widest log-area gap, or extrapolate below the deepest offset
-    hist = (np.array([0.0, -1.0]), None, None, np.array([0.5, 0.01]), None)
-    off = _next_offset(t, hist)
-    assert np.isfinite(off)
+def test_first_two_automatic_offsets():
+    top, mid, bot = 2.0, 0.5, -1.0
+    assert automatic_offset(0, top, mid, bot, [], [], 100) == -0.5
+    # just below the highest point: 1 % of the way from the top to the mean
+    assert automatic_offset(1, top, mid, bot, [-0.5], [0.3], 100) == pytest.approx(-2.0 + 0.015, abs=1e-15)
+
+
+def test_automatic_offset_bisects_the_widest_log_area_gap():
+    # areas 0.01 and 0.5 (N = 256): log10 gap 1.56 between them, 0.30 from
+    # 0.5 up to full contact -> the next step halves the first interval
+    off = automatic_offset(2, 2.0, 0.5, -1.0, [0.0, -1.0], [0.5, 0.01], 256)
+    assert off == pytest.approx(-0.5, abs=1e-15)
+    # order of the earlier steps does not matter
+    off2 = automatic_offset(2, 2.0, 0.5, -1.0, [-1.0, 0.0], [0.01, 0.5], 256)
+    assert off2 == off
+    # three steps: areas 0.001, 0.1, 0.11 -> widest gap between 0.001 and 0.1
+    off3 = automatic_offset(3, 2.0, 0.5, -1.0, [0.3, -0.2, -0.25], [0.001, 0.1, 0.11], 10 ** 6)
+    assert off3 == pytest.approx(0.05, abs=1e-15)
+
+
+def test_automatic_offset_extrapolates_below_the_deepest_step():
+    # areas 1e-3 and 2e-3 (N = 1e6): the widest interval is the one up to
+    # full contact -> go twice as far below the lowest point as the step with
+    # the largest area
+    off = automatic_offset(2, 2.0, 0.5, -1.0, [0.1, 0.2], [0.001, 0.002], 10 ** 6)
+    assert off == pytest.approx(-1.0 + 2 * (0.2 + 1.0), abs=1e-14)
