@@ -440,63 +440,91 @@ __device__ __forceinline__ void fft_line_to_last(cplx<T>* lds, int base, int str
 
 // ---------------------------------------------------------------------------
 // Reductions in FP64.  A reduction record holds NS slots; OPS[i] is the op.
-enum RedOp : int { RSUM = 0, RMAX = 1, RMIN = 2 };
+// RCSUM is a compensated sum: each thread accumulates with Neumaier's
+// correction term (kadd) and the tree combines (sum, correction) pairs with
+// Knuth's TwoSum (kcombine), so the result is the exactly rounded sum up to
+// O(n eps^2) of the sum of magnitudes.  RSUM is a plain sum, used for counts
+// (exact in FP64).
+enum RedOp : int { RSUM = 0, RMAX = 1, RMIN = 2, RCSUM = 3 };
 
 __device__ __forceinline__ double red_identity(int op) {
-  return op == RSUM ? 0.0 : (op == RMAX ? -__builtin_huge_val() : __builtin_huge_val());
+  return (op == RSUM || op == RCSUM) ? 0.0 : (op == RMAX ? -__builtin_huge_val() : __builtin_huge_val());
 }
 __device__ __forceinline__ double red_apply(int op, double a, double b) {
   // NaN-propagating max/min so that a NaN anywhere reaches the scalar logic
-  if (op == RSUM) return a + b;
+  if (op == RSUM || op == RCSUM) return a + b;
   if (a != a || b != b) return a + b;
   return op == RMAX ? (a > b ? a : b) : (a < b ? a : b);
 }
 
+// s + x with the rounding error of the addition collected in c (Neumaier).
+__device__ __forceinline__ void kadd(double& s, double& c, double x) {
+  const double t = s + x;
+  c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
+  s = t;
+}
+// (s1, c1) += (s2, c2); the rounding error of s1 + s2 is exact (TwoSum), and
+// the result does not depend on the order of the two operands.
+__device__ __forceinline__ void kcombine(double& s1, double& c1, double s2, double c2) {
+  const double t = s1 + s2;
+  const double bp = t - s1;
+  const double e = (s1 - (t - bp)) + (s2 - bp);
+  s1 = t;
+  c1 = (c1 + c2) + e;
+}
+// Value of a compensated pair; an infinite or NaN sum keeps its value.
+__device__ __forceinline__ double kvalue(double s, double c) { return (s - s == 0.0) ? s + c : s; }
+
 // Butterfly reduction over the lanes of one wavefront.  A block whose size is
 // not a multiple of 64 has a partial last wave: lanes past the block end do
 // not exist, so their (undefined) values are skipped.
-__device__ __forceinline__ double wave_reduce(int op, double v) {
+__device__ __forceinline__ void wave_reduce(int op, double& v, double& c) {
   const int lane = threadIdx.x & 63;
   const int nlanes = min(64, (int)blockDim.x - (int)(threadIdx.x & ~63u));
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     const double o = __shfl_xor(v, off, 64);
-    if ((lane ^ off) < nlanes) v = red_apply(op, v, o);
+    if (op == RCSUM) {
+      const double oc = __shfl_xor(c, off, 64);
+      if ((lane ^ off) < nlanes) kcombine(v, c, o, oc);
+    } else if ((lane ^ off) < nlanes) {
+      v = red_apply(op, v, o);
+    }
   }
-  return v;
 }
 
-// Block-wide reduction of NS slots; result valid in thread 0.
-// `scratch` must hold NS * (blockDim/64) doubles of shared memory.
+// Block-wide reduction of NS slots (vals) and the correction terms of the
+// compensated ones (comp); result valid in thread 0.  `scratch` must hold
+// 2 * NS * 16 doubles of shared memory (up to 16 waves).
 template <int NS>
-__device__ __forceinline__ void block_reduce(double* vals, const int* ops, double* scratch) {
+__device__ __forceinline__ void block_reduce(double* vals, double* comp, const int* ops, double* scratch) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
-    double v = wave_reduce(ops[i], vals[i]);
-    if (lane == 0) scratch[i * 16 + wid] = v;
+    double v = vals[i], c = comp[i];
+    wave_reduce(ops[i], v, c);
+    if (lane == 0) {
+      scratch[i * 16 + wid] = v;
+      if (ops[i] == RCSUM) scratch[(NS + i) * 16 + wid] = c;
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
       double v = scratch[i * 16];
-      for (int w = 1; w < nw; ++w) v = red_apply(ops[i], v, scratch[i * 16 + w]);
+      double c = ops[i] == RCSUM ? scratch[(NS + i) * 16] : 0.0;
+      for (int w = 1; w < nw; ++w) {
+        if (ops[i] == RCSUM)
+          kcombine(v, c, scratch[i * 16 + w], scratch[(NS + i) * 16 + w]);
+        else
+          v = red_apply(ops[i], v, scratch[i * 16 + w]);
+      }
       vals[i] = v;
+      comp[i] = c;
     }
   }
   __syncthreads();
 }
-
-// Neumaier-compensated accumulation for per-thread sums.
-struct Kahan {
-  double s = 0.0, c = 0.0;
-  __device__ __forceinline__ void add(double x) {
-    double t = s + x;
-    if (fabs(s) >= fabs(x)) c += (s - t) + x; else c += (x - t) + s;
-    s = t;
-  }
-  __device__ __forceinline__ double value() const { return s + c; }
-};
 
 }  // namespace cm

@@ -153,21 +153,29 @@ constexpr int kRefreshEvery = 8;
 // stored with agent-scope (L2 write-through) atomic stores, the wave waits for
 // them to complete (s_waitcnt vmcnt(0)) before it takes its ticket, and only
 // the last block invalidates its L2 (agent-scope acquire) before reading them.
+//
+// acc holds 2 NS doubles: the slot values, then the correction terms of the
+// compensated sums (RCSUM); a compensated slot's partial is the pair.
 template <class P>
 __device__ __forceinline__ bool reduce_blocks(double* acc, double* part, unsigned* ctr, int nblocks, int bid) {
   constexpr int NS = P::NS;
-  __shared__ double scr[8 * 16];
+  __shared__ double scr[2 * 8 * 16];
   __shared__ int s_last;
   int ops[NS > 0 ? NS : 1];
 #pragma unroll
   for (int i = 0; i < NS; ++i) ops[i] = P::op(i);
-  block_reduce<NS>(acc, ops, scr);
-  // partials are slot-major, part[i * nblocks + b], so the last block reads
-  // each slot with coalesced loads
+  block_reduce<NS>(acc, acc + NS, ops, scr);
+  // partials are slot-major, part[i * nblocks + b] (corrections of slot i at
+  // part[(NS + i) * nblocks + b]), so the last block reads each slot with
+  // coalesced loads
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int i = 0; i < NS; ++i)
+    for (int i = 0; i < NS; ++i) {
       __hip_atomic_store(&part[(size_t)i * nblocks + bid], acc[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (ops[i] == RCSUM)
+        __hip_atomic_store(&part[(size_t)(NS + i) * nblocks + bid], acc[NS + i], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
     wait_stores_done();
     const int g = bid % kTicketGroups;
     const int gsize = nblocks / kTicketGroups + (g < nblocks % kTicketGroups ? 1 : 0);
@@ -183,30 +191,42 @@ __device__ __forceinline__ bool reduce_blocks(double* acc, double* part, unsigne
   __syncthreads();
   if (!s_last) return false;
   acquire_agent();
-  double v[NS > 0 ? NS : 1];
+  double v[NS > 0 ? NS : 1], vc[NS > 0 ? NS : 1];
 #pragma unroll
-  for (int i = 0; i < NS; ++i) v[i] = red_identity(ops[i]);
+  for (int i = 0; i < NS; ++i) {
+    v[i] = red_identity(ops[i]);
+    vc[i] = 0.0;
+  }
   // plain loads (the acquire above makes every block's partials visible),
   // kB blocks per thread and slot issued together so they are all in flight
   constexpr int kB = 4;
   for (int b0 = threadIdx.x; b0 < nblocks; b0 += kB * blockDim.x) {
-    double w[NS > 0 ? NS : 1][kB];
+    double w[NS > 0 ? NS : 1][kB], wc[NS > 0 ? NS : 1][kB];
 #pragma unroll
     for (int i = 0; i < NS; ++i)
 #pragma unroll
       for (int k = 0; k < kB; ++k) {
         const int b = b0 + k * (int)blockDim.x;
         w[i][k] = b < nblocks ? part[(size_t)i * nblocks + b] : red_identity(ops[i]);
+        wc[i][k] = (ops[i] == RCSUM && b < nblocks) ? part[(size_t)(NS + i) * nblocks + b] : 0.0;
       }
 #pragma unroll
     for (int i = 0; i < NS; ++i)
 #pragma unroll
-      for (int k = 0; k < kB; ++k) v[i] = red_apply(ops[i], v[i], w[i][k]);
+      for (int k = 0; k < kB; ++k) {
+        if (ops[i] == RCSUM)
+          kcombine(v[i], vc[i], w[i][k], wc[i][k]);
+        else
+          v[i] = red_apply(ops[i], v[i], w[i][k]);
+      }
   }
-  block_reduce<NS>(v, ops, scr);
+  block_reduce<NS>(v, vc, ops, scr);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int i = 0; i < NS; ++i) acc[i] = v[i];
+    for (int i = 0; i < NS; ++i) {
+      acc[i] = v[i];
+      acc[NS + i] = vc[i];
+    }
     __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   return true;
@@ -374,7 +394,7 @@ template <class P>
 __device__ __forceinline__ void finalize_copy(const P& p, const double* acc) {
   double fin[P::NS > 0 ? P::NS : 1];
 #pragma unroll
-  for (int i = 0; i < P::NS; ++i) fin[i] = acc[i];
+  for (int i = 0; i < P::NS; ++i) fin[i] = P::op(i) == RCSUM ? kvalue(acc[i], acc[P::NS + i]) : acc[i];
   p.finalize(fin);
 }
 
@@ -397,9 +417,12 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_r2c(Geom g, LinePlan lp
   const int N2 = g.N2;
   const int r = threadIdx.x / lp.tpl, tl = threadIdx.x - r * lp.tpl;
   const int row = blockIdx.x * g.R + r;
-  double acc[P::NS > 0 ? P::NS : 1];
+  double acc[P::NS > 0 ? 2 * P::NS : 1];  // slot values, then corrections of compensated sums
 #pragma unroll
-  for (int i = 0; i < P::NS; ++i) acc[i] = red_identity(P::op(i));
+  for (int i = 0; i < P::NS; ++i) {
+    acc[i] = red_identity(P::op(i));
+    acc[P::NS + i] = 0.0;
+  }
   const int rb = r * N2;
   auto L = [&](int n) -> cplx<T>& { return lds[lpad<T>(rb + n)]; };
   // the producer feeds the first FFT pass directly (first-pass order)
@@ -472,7 +495,7 @@ __device__ void tau_finalize(Scal* s, double sum_rt);
 // Reduction slots of the spectral tau pass (one sum).
 struct TauRed {
   static constexpr int NS = 1;
-  __device__ static constexpr int op(int) { return RSUM; }
+  __device__ static constexpr int op(int) { return RCSUM; }
 };
 
 // Column pass.  MODE 0: forward * G * inverse; 1: forward only; 2: inverse
@@ -502,7 +525,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
     //   sum_x r(x) t(x) = sum_k G(k) |T(k)|^2
     // over the full spectrum: columns 0 < ky < My/2 count twice (Hermitian
     // half), the packed DC and Nyquist columns once each.
-    double acc = 0.0;
+    double acc[2] = {0.0, 0.0};  // compensated sum
     const int c = threadIdx.x % Cb, tl = threadIdx.x / Cb;
     if (cb != 0) {
       const int R0 = lp.radix(0), Rl = lp.rlast;
@@ -518,7 +541,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
       for (int e = 0; e < EMAX; ++e)
         if (e / Rl < nbl) {
           const double a = (double)v[e].x, b = (double)v[e].y;
-          acc += 2.0 * (double)Gk[slot_element(e, Rl, Mx, tl, lp.tpl)] * (a * a + b * b);
+          kadd(acc[0], acc[1], 2.0 * (double)Gk[slot_element(e, Rl, Mx, tl, lp.tpl)] * (a * a + b * b));
         }
     } else {
       slab_load(lds, S, base, n);
@@ -528,7 +551,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
         const T* Gk = G + (size_t)c * Mx;
         for (int x = tl; x < Mx; x += lp.tpl) {
           const cplx<T> q = lds[lpad<T>(x * Cb + c)];
-          acc += 2.0 * (double)Gk[x] * ((double)q.x * q.x + (double)q.y * q.y);
+          kadd(acc[0], acc[1], 2.0 * (double)Gk[x] * ((double)q.x * q.x + (double)q.y * q.y));
         }
       } else {
         // Q = A + iB: A(kx) = (Q(kx) + conj Q(-kx)) / 2 (ky = 0),
@@ -540,12 +563,12 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp
           const cplx<T> q = lds[lpad<T>(x * Cb)], qm = lds[lpad<T>(xm * Cb)];
           const double ar = 0.5 * ((double)q.x + qm.x), ai = 0.5 * ((double)q.y - qm.y);
           const double br = 0.5 * ((double)q.x - qm.x), bi = 0.5 * ((double)q.y + qm.y);
-          acc += (double)G0[x] * (ar * ar + ai * ai) + (double)GN[x] * (br * br + bi * bi);
+          kadd(acc[0], acc[1], (double)G0[x] * (ar * ar + ai * ai) + (double)GN[x] * (br * br + bi * bi));
         }
       }
     }
-    if (reduce_blocks<TauRed>(&acc, in.part, in.ctr + 4 * kCtrSlot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
-      tau_finalize(in.sc, acc);
+    if (reduce_blocks<TauRed>(acc, in.part, in.ctr + 4 * kCtrSlot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
+      tau_finalize(in.sc, kvalue(acc[0], acc[1]));
     return;
   }
   if (MODE == 0 && cb != 0) {
@@ -681,9 +704,12 @@ __global__ void __launch_bounds__(fft_block_max(EMAX)) k_c2r(Geom g, LinePlan lp
     }
   }
   __syncthreads();
-  double acc[P::NS > 0 ? P::NS : 1];
+  double acc[P::NS > 0 ? 2 * P::NS : 1];  // slot values, then corrections of compensated sums
 #pragma unroll
-  for (int i = 0; i < P::NS; ++i) acc[i] = red_identity(P::op(i));
+  for (int i = 0; i < P::NS; ++i) {
+    acc[i] = red_identity(P::op(i));
+    acc[P::NS + i] = 0.0;
+  }
   if constexpr (EMAX == 8) {
     // last inverse pass into registers and the consumer, specialised on the
     // last radix so every register slot index is a compile-time constant
@@ -730,9 +756,12 @@ __global__ void __launch_bounds__(256) k_elem(Geom g, InstArr<T> ia, int ctr_slo
   const Inst<T>& in = ia.get(blockIdx.y);
   P p(in);
   if (p.skip()) return;
-  double acc[P::NS > 0 ? P::NS : 1];
+  double acc[P::NS > 0 ? 2 * P::NS : 1];  // slot values, then corrections of compensated sums
 #pragma unroll
-  for (int i = 0; i < P::NS; ++i) acc[i] = red_identity(P::op(i));
+  for (int i = 0; i < P::NS; ++i) {
+    acc[i] = red_identity(P::op(i));
+    acc[P::NS + i] = 0.0;
+  }
   const int npair = g.My / 2;
   for (int row = blockIdx.x; row < g.Mx; row += gridDim.x) {
     for (int n0 = threadIdx.x; n0 < npair; n0 += kPairs * blockDim.x) {
@@ -931,21 +960,26 @@ __device__ void phase_a_logic(Scal* s, const double* a) {
 // forces (:239-249) and the sums of the gap g0 = u - h shifted by `shift` (the
 // offset of the previous iteration, which keeps sum (g - offset)^2 free of
 // cancellation); returns c.
+// acc: the six slots {A_contact, A_cg, Sd, Sdd, gmax, gmin}; cmp: their
+// correction terms (Sd and Sdd are compensated sums).
 __device__ __forceinline__ bool phase_a_point(bool has_H, double H, double shift, double fv, double g0, bool comp,
-                                              double* acc) {
+                                              double* acc, double* cmp) {
   const bool c0 = fv < 0.0;
   const bool c = c0 && (!has_H || fv > -H);
   acc[0] += c0 ? 1.0 : 0.0;
   acc[1] += c ? 1.0 : 0.0;
   if (comp) {
     const double d = g0 - shift;
-    if (c) acc[2] += d;                  // sum over the contact set only (:269)
-    acc[3] += (c ? 1.0 : 0.0) * d * d;   // c * g * g: a NaN gap poisons G as in :273
+    if (c) kadd(acc[2], cmp[2], d);                  // sum over the contact set only (:269)
+    kadd(acc[3], cmp[3], (c ? 1.0 : 0.0) * d * d);   // c * g * g: a NaN gap poisons G as in :273
     acc[4] = red_apply(RMAX, acc[4], g0);
     acc[5] = red_apply(RMIN, acc[5], g0);
   }
   return c;
 }
+
+// Reduction ops of the phase-A slots (shared by ElemPhaseA and ConsDisp).
+__device__ constexpr int phase_a_op(int i) { return i == 4 ? RMAX : (i == 5 ? RMIN : (i >= 2 ? RCSUM : RSUM)); }
 
 // Contact-mask bits: bit 0 = contact set c of the last finished iteration
 // (result.active_set), bit 1 = gap < 0 at the start of that iteration
@@ -964,7 +998,7 @@ template <typename T>
 struct ElemPhaseA {
   using F = T;
   static constexpr int NS = 6;  // A_contact, A_cg, Sd, Sdd, gmax, gmin
-  __device__ static constexpr int op(int i) { return i == 4 ? RMAX : (i == 5 ? RMIN : RSUM); }
+  __device__ static constexpr int op(int i) { return phase_a_op(i); }
   struct Rec {
     P2<double> f;
     P2<T> u;
@@ -981,8 +1015,8 @@ struct ElemPhaseA {
     const Scal* s = in.sc;
     const double w0 = r.h.c0 ? (double)r.u.a - r.h.h0 : (double)r.u.a;
     const double w1 = r.h.c1 ? (double)r.u.b - r.h.h1 : (double)r.u.b;
-    const bool c0 = phase_a_point(s->has_H, s->H, s->shift, r.f.a, w0, r.h.c0, acc);
-    const bool c1 = phase_a_point(s->has_H, s->H, s->shift, r.f.b, w1, r.h.c1, acc);
+    const bool c0 = phase_a_point(s->has_H, s->H, s->shift, r.f.a, w0, r.h.c0, acc, acc + NS);
+    const bool c1 = phase_a_point(s->has_H, s->H, s->shift, r.f.b, w1, r.h.c1, acc, acc + NS);
     const size_t p = (size_t)row * g.My + col;
     stm(in.cm + p, c0 ? kNext : 0, c1 ? kNext : 0);
     // FP64 plans write only where the gap differs from u (w aliases u)
@@ -1001,7 +1035,7 @@ __global__ void __launch_bounds__(1024) k_bisect(Geom g, InstArr<T> ia) {
   const Inst<T>& in = ia.get(blockIdx.x);
   Scal* s = in.sc;
   if (s->done || s->mode != MODE_MIX || !s->load_ctl) return;
-  __shared__ double scr[8 * 16];
+  __shared__ double scr[2 * 8 * 16];
   __shared__ double s_cnt;
   const size_t Mt = (size_t)g.nx * g.ny;
   const double H = s->H, F = s->F;
@@ -1014,7 +1048,8 @@ __global__ void __launch_bounds__(1024) k_bisect(Geom g, InstArr<T> ia) {
       v += (g0 - x < 0.0) ? 1.0 : 0.0;
     }
     int ops[1] = {RSUM};
-    block_reduce<1>(&v, ops, scr);
+    double unused = 0.0;
+    block_reduce<1>(&v, &unused, ops, scr);
     if (threadIdx.x == 0) s_cnt = v;
     __syncthreads();
     const double cnt = s_cnt;
@@ -1064,7 +1099,7 @@ template <typename T>
 struct ProdDir {
   using F = T;
   static constexpr int NS = 1;
-  __device__ static constexpr int op(int) { return RSUM; }
+  __device__ static constexpr int op(int) { return RCSUM; }
   struct Rec {
     P2<double> w;
     P2<T> t;
@@ -1080,13 +1115,14 @@ struct ProdDir {
     return Rec{ld2(in.w + p), ld2(in.t + p), ldm(in.cm + p), ldc(g, in.surf, row, col)};
   }
   // wv: the stored gap u - h
+  // Masks multiply, as in the reference (t = c (g + ...), sum c g t), so a
+  // NaN gap outside the contact set still reaches t and the sums.
   __device__ T one(const Scal* s, bool comp, bool c, double wv, T told, double* acc) const {
     if (!comp) return told;  // t is only ever set on comp (zero elsewhere)
     const double gap = wv - s->offset;
-    double tn = 0.0;
-    if (c) tn = s->use_beta ? gap + s->beta * (double)told : gap;
-    const T tt = (T)tn;
-    if (c) acc[0] += gap * (double)tt;
+    const double cm = c ? 1.0 : 0.0;
+    const T tt = (T)(cm * (s->use_beta ? gap + s->beta * (double)told : gap));
+    kadd(acc[0], acc[NS], cm * gap * (double)tt);
     return tt;
   }
   __device__ void make(const Geom& g, int row, int col, const Rec& r, T& a, T& b, double* acc) const {
@@ -1113,8 +1149,10 @@ __device__ void tau_finalize(Scal* s, double sum_rt) {
 
 // Per-pixel force after the PK or mixing update, before projection
 // (:296 and :322-323).  Shared by phase B and the second transform.
+// f + (tau c) t multiplies by the mask as the reference does (a NaN in t
+// reaches f even where c = 0).
 __device__ __forceinline__ double force_update(const Scal* s, double fv, double tv, bool c) {
-  if (s->mode == MODE_PK) return fv + (c ? s->tau * tv : 0.0);
+  if (s->mode == MODE_PK) return fv + (c ? s->tau : 0.0) * tv;
   return (1.0 - s->mix) * fv - (s->mix * s->H) * (c ? 1.0 : 0.0);
 }
 
@@ -1124,7 +1162,7 @@ struct ElemPhaseB {
   using F = T;
   static constexpr int NS = 7;  // n_tensile, max_tensile, pad_max, A_fl, min_fl, sum_f2, nc
   __device__ static constexpr int op(int i) {
-    return (i == 1 || i == 2) ? RMAX : (i == 4 ? RMIN : RSUM);
+    return (i == 1 || i == 2) ? RMAX : (i == 4 ? RMIN : (i == 5 ? RCSUM : RSUM));
   }
   struct Rec {
     P2<double> f;
@@ -1161,7 +1199,7 @@ struct ElemPhaseB {
       acc[4] = red_apply(RMIN, acc[4], f1 + s->H);
     }
     if (comp) {
-      acc[5] += tensile ? 0.0 : f1;
+      kadd(acc[5], acc[NS + 5], tensile ? 0.0 : f1);
       const bool nc = (tensile && gap < 0.0) || (flowing && gap > 0.0);
       acc[6] += nc ? 1.0 : 0.0;
     }
@@ -1215,7 +1253,7 @@ template <typename T, typename TT, bool INC>
 struct ProdForce {
   using F = T;
   static constexpr int NS = 1;  // sum f over comp
-  __device__ static constexpr int op(int) { return RSUM; }
+  __device__ static constexpr int op(int) { return RCSUM; }
   struct Rec {
     P2<double> f;
     P2<T> t;
@@ -1253,7 +1291,7 @@ struct ProdForce {
       if (s->do_nc && nc) fo = fo + s->tau * gap;         // :384
       cmv = (uint8_t)((c ? kCur | kNext : 0) | (gap < 0.0 ? kGneg : 0));
     }
-    if (comp) acc[0] += fo;
+    if (comp) kadd(acc[0], acc[NS], fo);
     return fo;
   }
   __device__ void make(const Geom& g, int row, int col, const Rec& r, TT& a, TT& b, double* acc) const {
@@ -1294,9 +1332,7 @@ struct ConsDisp {
   // two output pairs per batch: the record by value and the transform's
   // registers leave no room for four under 128 VGPRs (four waves per SIMD)
   static constexpr int kBatch = 2;
-  __device__ static constexpr int op(int i) {
-    return i < 2 ? RMAX : (i == 6 ? RMAX : (i == 7 ? RMIN : RSUM));
-  }
+  __device__ static constexpr int op(int i) { return i < 2 ? RMAX : phase_a_op(i - 2); }
   struct Rec {
     P2<double> w, f;
     P2<uint8_t> cm;
@@ -1341,7 +1377,7 @@ struct ConsDisp {
       acc[1] = red_apply(RMAX, acc[1], pen);
     }
     // next shift = this offset
-    const bool cn = phase_a_point(cx.has_H, cx.H, cx.offset, fv, wn, comp, acc + 2);
+    const bool cn = phase_a_point(cx.has_H, cx.H, cx.offset, fv, wn, comp, acc + 2, acc + NS + 2);
     return (uint8_t)((cmv & (kCur | kGneg)) | (cn ? kNext : 0));
   }
   __device__ void take(const Geom& g, int row, int col, const Rec& r, TT a, TT b, double* acc, const Ctx& cx) const {
@@ -1408,7 +1444,7 @@ template <typename T>
 struct ElemFinish {
   using F = T;
   static constexpr int NS = 1;
-  __device__ static constexpr int op(int) { return RSUM; }
+  __device__ static constexpr int op(int) { return RCSUM; }
   struct Rec {
     P2<double> f, w;
     HPair h;
@@ -1430,8 +1466,8 @@ struct ElemFinish {
       st2(in.u + p, (T)u0, (T)u1);
       st2(in.f + p, (T)r.f.a, (T)r.f.b);
     }
-    if (row < g.nx && col < g.ny) acc[0] += r.f.a * u0;
-    if (row < g.nx && col + 1 < g.ny) acc[0] += r.f.b * u1;
+    if (row < g.nx && col < g.ny) kadd(acc[0], acc[NS], r.f.a * u0);
+    if (row < g.nx && col + 1 < g.ny) kadd(acc[0], acc[NS], r.f.b * u1);
   }
   __device__ void finalize(double* a) const { in.sc->fun = -a[0] / 2; }
 };
@@ -2098,12 +2134,12 @@ size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
 WsLayout ws_layout(const cm_plan* P, int B) {
   const size_t w = P->prec == CM_FP64 ? 8 : 4;
   const size_t M = (size_t)P->Mx * P->My;
-  // partials: up to 8 slots per block of any reducing launch (element-wise,
-  // row passes, column pass) of either executor
+  // partials: up to 8 slots and their 8 correction terms per block of any
+  // reducing launch (element-wise, row passes, column pass) of either executor
   const size_t nblk_max = std::max<size_t>({(size_t)P->nblk_elem, (size_t)(P->Mx / P->ex.g.R),
                                             (size_t)(P->ex.g.N2 / P->ex.g.Cb), (size_t)(P->Mx / P->ex64.g.R),
                                             (size_t)(P->ex64.g.N2 / P->ex64.g.Cb)});
-  const size_t nparts = nblk_max * 8;
+  const size_t nparts = nblk_max * 16;
   WsLayout L{};
   L.st_t = al256(M * w);
   L.st_S = al256((M / 2) * 2 * w);
@@ -2225,6 +2261,26 @@ void enqueue_iteration(cm_session* S, bool refresh) {
 
 void session_free(cm_session* S);
 
+// Launch shapes, workspace layout and stream of a session of B instances
+// (no allocation).
+template <typename T>
+void session_configure(cm_session* S, cm_plan* P, int B, const cm_ccg_params* prm, void* stream) {
+  S->P = P;
+  S->ex = P->ex;
+  S->ex64 = std::is_same<T, double>::value ? P->ex : P->ex64;
+  S->refresh_every = prm[0].refresh_every > 0 ? prm[0].refresh_every : kRefreshEvery;
+  S->B = B;
+  S->st = stream ? (hipStream_t)stream : P->ctx->stream;
+  S->L = ws_layout(P, B);
+  S->use_graph = prm[0].use_graph;
+}
+
+// Instance tables and scalar records (host buffers S->hin, S->hin0, S->host)
+// and the set-up launches on the workspace S->ws; allocates nothing.
+template <typename T>
+int session_init(cm_session* S, const cm_ccg_params* prm, const double* const* hs, const uint8_t* const* surfs,
+                 void* const* us, void* const* fs, uint8_t* const* cms);
+
 template <typename T>
 int session_begin(cm_plan* P, int B, const cm_ccg_params* prm, const double* const* hs,
                   const uint8_t* const* surfs, void* const* us, void* const* fs, uint8_t* const* cms,
@@ -2234,13 +2290,7 @@ int session_begin(cm_plan* P, int B, const cm_ccg_params* prm, const double* con
     if (prm[b].init_mode != prm[0].init_mode)
       return fail(ctx, CM_E_ARG, "all instances of a batch must share init_mode");
   cm_session* S = new cm_session();
-  S->P = P;
-  S->ex = P->ex;
-  S->ex64 = std::is_same<T, double>::value ? P->ex : P->ex64;
-  S->refresh_every = prm[0].refresh_every > 0 ? prm[0].refresh_every : kRefreshEvery;
-  S->B = B;
-  S->st = stream ? (hipStream_t)stream : ctx->stream;
-  S->L = ws_layout(P, B);
+  session_configure<T>(S, P, B, prm, stream);
   int rc = dev_alloc(ctx, &S->ws, S->L.total);
   if (rc) {
     delete S;
@@ -2250,17 +2300,30 @@ int session_begin(cm_plan* P, int B, const cm_ccg_params* prm, const double* con
   if (hipHostMalloc((void**)&S->host, sizeof(Scal) * B, 0) != hipSuccess ||
       hipHostMalloc(&S->hin, sizeof(Inst<T>) * B, 0) != hipSuccess ||
       hipHostMalloc(&S->hin0, sizeof(Inst<T>) * B, 0) != hipSuccess) {
-    dev_free(ctx, S->ws, S->ws_bytes);
-    delete S;
+    session_free(S);
     return fail(ctx, CM_E_OOM, "pinned host allocation failed");
   }
+  if ((rc = session_init<T>(S, prm, hs, surfs, us, fs, cms))) {
+    session_free(S);
+    return rc;
+  }
+  *out = S;
+  return 0;
+}
+
+template <typename T>
+int session_init(cm_session* S, const cm_ccg_params* prm, const double* const* hs, const uint8_t* const* surfs,
+                 void* const* us, void* const* fs, uint8_t* const* cms) {
+  cm_plan* P = S->P;
+  cm_ctx* ctx = P->ctx;
+  const int B = S->B;
+  int rc = 0;
   const WsLayout& L = S->L;
   char* base = (char*)S->ws;
   const size_t M = (size_t)P->Mx * P->My;
   Inst<T>* hin = (Inst<T>*)S->hin;
   Inst<T>* hin0 = (Inst<T>*)S->hin0;
   Scal* hsc = S->host;
-  S->use_graph = prm[0].use_graph;
   for (int b = 0; b < B; ++b) {
     const cm_ccg_params& q = prm[b];
     Inst<T>& I = hin[b];
@@ -2350,12 +2413,7 @@ int session_begin(cm_plan* P, int B, const cm_ccg_params* prm, const double* con
     launch_elem<T, ElemZeroPad<T>>(ex, nbe, ia, B, 1, st);
   }
   launch_elem<T, ElemPhaseA<T>>(ex, nbe, ia, B, 2, st);  // start of iteration 1
-  if ((rc = launch_status(ctx))) {
-    session_free(S);
-    return rc;
-  }
-  *out = S;
-  return 0;
+  return launch_status(ctx);
 }
 
 // Enqueue n iterations; instances that have finished turn the kernels into

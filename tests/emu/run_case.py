@@ -89,5 +89,5 @@ def operators(case):
 
 if __name__ == "__main__":
     case = json.loads(sys.argv[1])
-    out = solve(case) if case["kind"] == "solve" else operators(case)
+    out = {"solve": solve, "ops": operators}[case["kind"]](case)
     print(json.dumps(out), flush=True)

@@ -129,10 +129,22 @@ def test_maxiter_nan_and_callback(gpu):
                   callback=lambda it, f, d: seen.append((it, d["area"])))
     assert res.success
     assert [s[0] for s in seen] == list(range(1, res.nit))
+    # the iteration that reaches maxiter and the NaN iteration also reach the
+    # callback (reference :478-492), as in the oracle
+    seen_g, seen_c = [], []
+    ccg_gpu(sub, h, external_force=0.01, pentol=1e-12, maxiter=3, callback=lambda it, f, d: seen_g.append(it))
+    ccg_cpu(PeriodicHalfSpace((n, n), 1.0, (1.0, 1.0)), h, external_force=0.01, pentol=1e-12, maxiter=3,
+            callback=lambda it, f, d: seen_c.append(it))
+    assert seen_g == seen_c == [1, 2, 3]
     bad = h.copy()
     bad[3, 3] = np.nan
+    nan_g, nan_c = [], []
     with pytest.raises(RuntimeError, match="nan encountered"):
-        ccg_gpu(sub, bad, external_force=0.01, pentol=1e-12)
+        ccg_gpu(sub, bad, external_force=0.01, pentol=1e-12, callback=lambda it, f, d: nan_g.append(it))
+    with pytest.raises(RuntimeError, match="nan encountered"):
+        ccg_cpu(PeriodicHalfSpace((n, n), 1.0, (1.0, 1.0)), bad, external_force=0.01, pentol=1e-12,
+                callback=lambda it, f, d: nan_c.append(it))
+    assert nan_g == nan_c and len(nan_g) >= 1
 
 
 def test_plastic_cone_notebook_first_load(gpu):
@@ -214,7 +226,10 @@ def test_fp32_solve_matches_oracle(gpu, bc, hard):
     assert l2 < 1e-6 and rel(rg.jac, rc.jac) < 1e-6
     assert np.sum((rg.jac > 0) != (rc.jac > 0)) <= 2
     np.testing.assert_allclose(rg.offset, rc.offset, rtol=1e-7)
-    assert abs(rg.nit - rc.nit) <= max(3, rc.nit // 20)
+    # elastic solves take the oracle's iteration count up to a few; with a
+    # hardness the mixing steps' bisection counts pixels, so round-off can
+    # shift when the solver switches between mixing and PK steps
+    assert abs(rg.nit - rc.nit) <= (max(3, rc.nit // 20) if not hard else max(8, rc.nit // 5))
 
 
 @pytest.mark.parametrize("bc", ["periodic", "free"])
@@ -260,3 +275,28 @@ def test_full_size_iterations_match_oracle(gpu, bc):
     np.testing.assert_allclose(rg.offset, rc.offset, rtol=tol)
     flips = int(np.sum(rg.active_set != rc.active_set))
     assert flips == 0 if bc == "periodic" else flips <= 2
+
+
+def test_compensated_sums_match_fsum(gpu):
+    """The FP64 sums of the solver are compensated (Neumaier per thread,
+    TwoSum across lanes and blocks): the total force and the elastic energy
+    of a converged solve equal math.fsum of the same terms to within 2 eps of
+    their magnitude sum."""
+    import math
+
+    from contactmechanics_amd.session import CcgSession, make_params
+    n = 256
+    h = rough(n, 41)
+    sub = PeriodicFFTElasticHalfSpace((n, n), 1.0, (1.0, 1.0), precision="fp64")
+    s = CcgSession(sub, [h], [make_params(external_force=0.01, pentol=1e-10, forcetol=1e-14, maxiter=5000)])
+    st = s.run()[0]
+    assert st["status"] == 0
+    f = s.f[0].cpu().numpy()[:n, :n].ravel()
+    u = s.u[0].cpu().numpy()[:n, :n].ravel()
+    s.close()
+    eps = np.finfo(np.float64).eps
+    tf = -math.fsum(f)
+    assert abs(st["total_force"] - tf) <= 2 * eps * math.fsum(np.abs(f)), (st["total_force"], tf)
+    prods = f * u  # the same IEEE products the kernel forms
+    fun = -math.fsum(prods) / 2
+    assert abs(st["fun"] - fun) <= 2 * eps * math.fsum(np.abs(prods)) / 2, (st["fun"], fun)
