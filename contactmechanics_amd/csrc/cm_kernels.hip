@@ -192,33 +192,33 @@ __device__ __forceinline__ bool reduce_blocks(double* acc, double* part, unsigne
   if (!s_last) return false;
   acquire_agent();
   double v[NS > 0 ? NS : 1], vc[NS > 0 ? NS : 1];
+  // one slot at a time (only that slot's batch of partials is live: the
+  // last block's registers count for every block of the kernel), kB blocks
+  // per thread issued together so their loads are in flight at once; plain
+  // loads, the acquire above makes every block's partials visible
+  constexpr int kB = 4;
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
-    v[i] = red_identity(ops[i]);
-    vc[i] = 0.0;
-  }
-  // plain loads (the acquire above makes every block's partials visible),
-  // kB blocks per thread and slot issued together so they are all in flight
-  constexpr int kB = 4;
-  for (int b0 = threadIdx.x; b0 < nblocks; b0 += kB * blockDim.x) {
-    double w[NS > 0 ? NS : 1][kB], wc[NS > 0 ? NS : 1][kB];
-#pragma unroll
-    for (int i = 0; i < NS; ++i)
+    const bool cs = ops[i] == RCSUM;
+    double vi = red_identity(ops[i]), ci = 0.0;
+    for (int b0 = threadIdx.x; b0 < nblocks; b0 += kB * blockDim.x) {
+      double w[kB], wc[kB];
 #pragma unroll
       for (int k = 0; k < kB; ++k) {
         const int b = b0 + k * (int)blockDim.x;
-        w[i][k] = b < nblocks ? part[(size_t)i * nblocks + b] : red_identity(ops[i]);
-        wc[i][k] = (ops[i] == RCSUM && b < nblocks) ? part[(size_t)(NS + i) * nblocks + b] : 0.0;
+        w[k] = b < nblocks ? part[(size_t)i * nblocks + b] : red_identity(ops[i]);
+        wc[k] = (cs && b < nblocks) ? part[(size_t)(NS + i) * nblocks + b] : 0.0;
       }
-#pragma unroll
-    for (int i = 0; i < NS; ++i)
 #pragma unroll
       for (int k = 0; k < kB; ++k) {
-        if (ops[i] == RCSUM)
-          kcombine(v[i], vc[i], w[i][k], wc[i][k]);
+        if (cs)
+          kcombine(vi, ci, w[k], wc[k]);
         else
-          v[i] = red_apply(ops[i], v[i], w[i][k]);
+          vi = red_apply(ops[i], vi, w[k]);
       }
+    }
+    v[i] = vi;
+    vc[i] = ci;
   }
   block_reduce<NS>(v, vc, ops, scr);
   if (threadIdx.x == 0) {
@@ -403,10 +403,14 @@ __device__ __forceinline__ void finalize_copy(const P& p, const double* acc) {
 // 16/32-element builds (up to 256 VGPRs) one, and they serve the long lines
 // (up to 8192 points, e.g. 8000 = 16*4*5^3) that need more than 256 threads.
 __host__ __device__ constexpr int fft_block_max(int) { return 512; }
+// Waves per SIMD the compiler must leave room for: four for the 8-element
+// builds (two 512-thread blocks per CU at <= 128 VGPRs; without the bound the
+// column pass took 153 VGPRs and three waves), one for the long-line builds.
+__host__ __device__ constexpr int fft_min_waves(int emax) { return emax <= 8 ? 4 : 1; }
 
 // Row pass: real rows -> packed half spectrum (forward, unnormalised).
 template <typename T, int EMAX, class P>
-__global__ void __launch_bounds__(fft_block_max(EMAX)) k_r2c(Geom g, LinePlan lp, const cplx<T>* __restrict__ tw,
+__global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_r2c(Geom g, LinePlan lp, const cplx<T>* __restrict__ tw,
                                              const cplx<T>* __restrict__ twp, InstArr<typename P::F> ia,
                                              int ctr_slot) {
   extern __shared__ __align__(16) unsigned char smem_r2c[];
@@ -504,7 +508,7 @@ struct TauRed {
 // skip: 0 never, 1 when the solve is done, 2 when done or in a mixing step.
 // T: transform precision; TF: field precision of the instances.
 template <typename T, int EMAX, int MODE, typename TF>
-__global__ void __launch_bounds__(fft_block_max(EMAX)) k_col(Geom g, LinePlan lp, LinePlan rv, const cplx<T>* __restrict__ tw,
+__global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_col(Geom g, LinePlan lp, LinePlan rv, const cplx<T>* __restrict__ tw,
                                               InstArr<TF> ia, int useK, int skip) {
   extern __shared__ __align__(16) unsigned char smem_col[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_col);
@@ -660,7 +664,7 @@ __device__ __forceinline__ void c2r_finish(const P& p, const Geom& g, int row, i
 // Row pass: packed half spectrum -> real rows (inverse, unnormalised; the
 // round trip r2c -> c2r scales by Mx*My, which the kernels G/K absorb).
 template <typename T, int EMAX, class P>
-__global__ void __launch_bounds__(fft_block_max(EMAX)) k_c2r(Geom g, LinePlan lp, const cplx<T>* __restrict__ tw,
+__global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_c2r(Geom g, LinePlan lp, const cplx<T>* __restrict__ tw,
                                              const cplx<T>* __restrict__ twp, InstArr<typename P::F> ia,
                                              int ctr_slot) {
   extern __shared__ __align__(16) unsigned char smem_c2r[];
@@ -960,18 +964,21 @@ __device__ void phase_a_logic(Scal* s, const double* a) {
 // forces (:239-249) and the sums of the gap g0 = u - h shifted by `shift` (the
 // offset of the previous iteration, which keeps sum (g - offset)^2 free of
 // cancellation); returns c.
-// acc: the six slots {A_contact, A_cg, Sd, Sdd, gmax, gmin}; cmp: their
-// correction terms (Sd and Sdd are compensated sums).
+// acc: the six slots {A_contact, A_cg, Sd, Sdd, gmax, gmin}.  Sd and Sdd are
+// sums of the shifted gap d = g - shift, centred on the previous offset, so
+// they carry no cancellation of large terms; they are plain FP64 sums (a
+// compensated pair in the displacement pass, where they are fused, took that
+// kernel from 122 VGPRs to 237, or 380 spills at four waves per SIMD).
 __device__ __forceinline__ bool phase_a_point(bool has_H, double H, double shift, double fv, double g0, bool comp,
-                                              double* acc, double* cmp) {
+                                              double* acc) {
   const bool c0 = fv < 0.0;
   const bool c = c0 && (!has_H || fv > -H);
   acc[0] += c0 ? 1.0 : 0.0;
   acc[1] += c ? 1.0 : 0.0;
   if (comp) {
     const double d = g0 - shift;
-    if (c) kadd(acc[2], cmp[2], d);                  // sum over the contact set only (:269)
-    kadd(acc[3], cmp[3], (c ? 1.0 : 0.0) * d * d);   // c * g * g: a NaN gap poisons G as in :273
+    if (c) acc[2] += d;                  // sum over the contact set only (:269)
+    acc[3] += (c ? 1.0 : 0.0) * d * d;   // c * g * g: a NaN gap poisons G as in :273
     acc[4] = red_apply(RMAX, acc[4], g0);
     acc[5] = red_apply(RMIN, acc[5], g0);
   }
@@ -979,7 +986,7 @@ __device__ __forceinline__ bool phase_a_point(bool has_H, double H, double shift
 }
 
 // Reduction ops of the phase-A slots (shared by ElemPhaseA and ConsDisp).
-__device__ constexpr int phase_a_op(int i) { return i == 4 ? RMAX : (i == 5 ? RMIN : (i >= 2 ? RCSUM : RSUM)); }
+__device__ constexpr int phase_a_op(int i) { return i == 4 ? RMAX : (i == 5 ? RMIN : RSUM); }
 
 // Contact-mask bits: bit 0 = contact set c of the last finished iteration
 // (result.active_set), bit 1 = gap < 0 at the start of that iteration
@@ -1015,8 +1022,8 @@ struct ElemPhaseA {
     const Scal* s = in.sc;
     const double w0 = r.h.c0 ? (double)r.u.a - r.h.h0 : (double)r.u.a;
     const double w1 = r.h.c1 ? (double)r.u.b - r.h.h1 : (double)r.u.b;
-    const bool c0 = phase_a_point(s->has_H, s->H, s->shift, r.f.a, w0, r.h.c0, acc, acc + NS);
-    const bool c1 = phase_a_point(s->has_H, s->H, s->shift, r.f.b, w1, r.h.c1, acc, acc + NS);
+    const bool c0 = phase_a_point(s->has_H, s->H, s->shift, r.f.a, w0, r.h.c0, acc);
+    const bool c1 = phase_a_point(s->has_H, s->H, s->shift, r.f.b, w1, r.h.c1, acc);
     const size_t p = (size_t)row * g.My + col;
     stm(in.cm + p, c0 ? kNext : 0, c1 ? kNext : 0);
     // FP64 plans write only where the gap differs from u (w aliases u)
@@ -1377,7 +1384,7 @@ struct ConsDisp {
       acc[1] = red_apply(RMAX, acc[1], pen);
     }
     // next shift = this offset
-    const bool cn = phase_a_point(cx.has_H, cx.H, cx.offset, fv, wn, comp, acc + 2, acc + NS + 2);
+    const bool cn = phase_a_point(cx.has_H, cx.H, cx.offset, fv, wn, comp, acc + 2);
     return (uint8_t)((cmv & (kCur | kGneg)) | (cn ? kNext : 0));
   }
   __device__ void take(const Geom& g, int row, int col, const Rec& r, TT a, TT b, double* acc, const Ctx& cx) const {
