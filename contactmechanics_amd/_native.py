@@ -99,6 +99,8 @@ SIGNATURES = {
     "cm_evaluate_force": ([_VP, _VP, _VP, _VP], ctypes.c_int),
     "cm_ccg_solve": ([_VP, ctypes.POINTER(CcgParams), _VP, _VP, _VP, _VP, _VP,
                       ctypes.POINTER(CcgStats), PROGRESS_FN, _VP, _VP], ctypes.c_int),
+    "cm_ccg_solve_noalloc": ([_VP, ctypes.POINTER(CcgParams), _VP, _VP, _VP, _VP, _VP,
+                              ctypes.POINTER(CcgStats), _VP], ctypes.c_int),
     "cm_ccg_solve_batch": ([_VP, ctypes.c_int, ctypes.POINTER(CcgParams), ctypes.POINTER(_VP),
                             ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(_VP),
                             ctypes.POINTER(_VP), ctypes.POINTER(CcgStats), _VP], ctypes.c_int),

@@ -1525,11 +1525,21 @@ __global__ void k_unpack_green(Geom g, const cplx<double>* S, double* out) {
 
 using namespace cm;
 
+// Device scratch of the allocation-free solve (cm_ccg_solve_noalloc): the
+// whole workspace of one solve -- conjugate direction, spectra, FP64 gap and
+// forces of FP32 plans, scalar record, partial reductions, tickets, instance
+// tables -- lives in this file-scope array.  Every GPU the library runs on
+// holds its own copy until the process ends; host code reaches it only through
+// the address hipGetSymbolAddress returns for the current device (cm_ctx).
+constexpr size_t kScratchBytes = (size_t)136 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 struct cm_ctx {
   int device = 0;
   size_t budget = 0;
   size_t used = 0;
   hipStream_t stream = nullptr;
+  void* scratch = nullptr;  // device address of `scratch` on this context's device
   std::string err;
   std::mutex mu;
 };
@@ -2607,6 +2617,66 @@ int solve_driver(cm_plan* P, int B, const cm_ccg_params* prm, const double* cons
   return result;
 }
 
+// Allocation-free solve of one instance: the workspace is this device's
+// static scratch, the host-side records live on the stack, and the solve runs
+// without graphs (instantiating one allocates).  Same kernels, same order as
+// solve_driver, hence the same results.
+template <typename T>
+int solve_noalloc(cm_plan* P, const cm_ccg_params* prm, const double* h, const uint8_t* surf, void* u, void* f,
+                  uint8_t* cm, cm_ccg_stats* stats, void* stream) {
+  cm_ctx* ctx = P->ctx;
+  if (!ctx->scratch) return fail(ctx, CM_E_HIP, "static scratch has no device address");
+  cm_ccg_params q = *prm;
+  q.use_graph = 0;
+  cm_session S;
+  session_configure<T>(&S, P, 1, &q, stream);
+  if (S.L.total > kScratchBytes)
+    return fail(ctx, CM_E_OOM,
+                "solve workspace of " + std::to_string(S.L.total) + " bytes exceeds the static scratch of " +
+                    std::to_string(kScratchBytes) + " bytes of cm_ccg_solve_noalloc");
+  Scal host{};
+  Inst<T> hin{}, hin0{};
+  S.ws = ctx->scratch;
+  S.ws_bytes = 0;  // not owned
+  S.host = &host;
+  S.hin = &hin;
+  S.hin0 = &hin0;
+  const double* hs[1] = {h};
+  const uint8_t* ms[1] = {surf};
+  void* us[1] = {u};
+  void* fs[1] = {f};
+  uint8_t* cs[1] = {cm};
+  int rc = session_init<T>(&S, &q, hs, ms, us, fs, cs);
+  int result = rc;
+  const int ips = q.iters_per_sync;
+  int chunk = ips > 0 ? ips : 4;
+  cm_ccg_stats st{};
+  while (result == 0) {
+    if ((rc = session_step<T>(&S, chunk))) {
+      result = rc;
+      break;
+    }
+    int done = 0;
+    if ((rc = session_poll(&S, &st, &done))) {
+      result = rc;
+      break;
+    }
+    if (done) break;
+    if (S.issued > (long long)S.maxiter + 8) {
+      result = fail(ctx, CM_E_ARG, "solver did not terminate within maxiter");
+      break;
+    }
+    if (ips <= 0) chunk = std::min(chunk * 2, 64);
+  }
+  if (result == 0) {
+    session_finish<T>(&S);
+    if ((rc = session_poll(&S, stats, nullptr))) result = rc;
+  }
+  (void)hipStreamSynchronize(S.st);  // the stack records must outlive every copy
+  if (result == 0) result = stats->status;
+  return result;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -2633,6 +2703,14 @@ int cm_ctx_create(int device, size_t mem_budget, cm_ctx** out) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return fail(nullptr, CM_E_HIP, "hipStreamCreate failed");
+  }
+  // this device's copy of the static scratch (the array's name is only a
+  // host-side stand-in: kernels and copies take this address)
+  hipError_t e = hipGetSymbolAddress(&c->scratch, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return fail(nullptr, CM_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
   }
   *out = c;
   return 0;
@@ -2761,6 +2839,14 @@ int cm_ccg_solve(cm_plan* plan, const cm_ccg_params* params, const double* heigh
   void* fs[1] = {f};
   uint8_t* cs[1] = {cmask};
   return solve_driver(plan, 1, params, hs, ms, us, fs, cs, stats, progress, user, stream);
+}
+
+int cm_ccg_solve_noalloc(cm_plan* plan, const cm_ccg_params* params, const double* heights, const uint8_t* surf_mask,
+                         void* u, void* f, uint8_t* cmask, cm_ccg_stats* stats, void* stream) {
+  if (!plan || !params || !heights || !u || !f || !cmask || !stats)
+    return fail(plan ? plan->ctx : nullptr, CM_E_ARG, "null argument");
+  return plan->prec == CM_FP64 ? solve_noalloc<double>(plan, params, heights, surf_mask, u, f, cmask, stats, stream)
+                               : solve_noalloc<float>(plan, params, heights, surf_mask, u, f, cmask, stats, stream);
 }
 
 int cm_ccg_solve_batch(cm_plan* plan, int B, const cm_ccg_params* params, const double* const* heights,

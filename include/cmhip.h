@@ -159,6 +159,20 @@ int cm_ccg_solve(cm_plan* plan, const cm_ccg_params* params, const double* heigh
                  const uint8_t* surf_mask, void* u, void* f, uint8_t* cmask,
                  cm_ccg_stats* stats, cm_progress_fn progress, void* user, void* stream);
 
+/* cm_ccg_solve without any allocation.  No counterpart in the reference: for
+ * callers that cannot allocate (serving loops with a fixed memory footprint,
+ * harnesses that forbid hipMalloc).  The whole workspace of the solve lives in
+ * a static device array of 136 MiB that every GPU the library runs on holds
+ * until the process ends; a solve whose workspace (cm_ccg_session_bytes(plan,
+ * 1)) exceeds it returns CM_E_OOM.  Calls on one GPU share that array, which
+ * keeps the last call's contents: calls that overlap in time on the same GPU
+ * (from any thread or context) race.  No progress callback; use_graph is
+ * ignored (the iterations are launched directly).  Same kernels and results
+ * as cm_ccg_solve. */
+int cm_ccg_solve_noalloc(cm_plan* plan, const cm_ccg_params* params, const double* heights,
+                         const uint8_t* surf_mask, void* u, void* f, uint8_t* cmask,
+                         cm_ccg_stats* stats, void* stream);
+
 /* Batched solve of B independent instances on the same plan (same grid and
  * elastic constants), one launch per kernel for all instances.  Arrays are
  * host arrays of B device pointers; stats is a host array of B records. */

@@ -147,6 +147,8 @@ extern hipError_t emu_last_error;
 inline hipError_t hipGetLastError() { hipError_t e = emu_last_error; emu_last_error = 0; return e; }
 inline hipError_t hipPeekAtLastError() { return emu_last_error; }
 inline hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int) { return hipSuccess; }
+#define HIP_SYMBOL(x) ((const void*)&(x))
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) { *p = (void*)sym; return hipSuccess; }
 inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return hipErrorNotSupported; }
 inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t*) { return hipErrorNotSupported; }
 inline hipError_t hipGraphInstantiate(hipGraphExec_t*, hipGraph_t, void*, void*, size_t) { return hipErrorNotSupported; }

@@ -67,6 +67,38 @@ def solve(case):
                 rel=float(np.max(np.abs(jac - r.jac)) / np.max(np.abs(r.jac))))
 
 
+def noalloc(case):
+    """cm_ccg_solve_noalloc against cm_ccg_solve: two calls in a row, the
+    second on a smaller grid; results must be bitwise equal."""
+    out = []
+    for nx, ny in case["grids"]:
+        p, mx, my = plan(nx, ny, case["bc"], case["prec"])
+        h = np.ascontiguousarray(heights(nx, ny, nx + ny))
+        dt = np.float64 if case["prec"] == 0 else np.float32
+        res = []
+        for entry in ("solve", "noalloc"):
+            u = np.zeros((mx, my), dt)
+            f = np.zeros((mx, my), dt)
+            cm = np.zeros((mx, my), np.uint8)
+            prm = N.CcgParams(load_control=1, external_force=3e-3, offset=float("nan"), has_hardness=0, hardness=0.0,
+                              pentol=1e-10, forcetol=1e-14, thermotol=1e-6, mixfac=0.1, mixdecfac=0.95,
+                              minmixsteps=10, maxiter=600, init_mode=0, iters_per_sync=0, use_graph=0,
+                              refresh_every=0)
+            st = N.CcgStats()
+            vp = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
+            if entry == "solve":
+                rc = lib.cm_ccg_solve(p, ctypes.byref(prm), vp(h), None, vp(u), vp(f), vp(cm), ctypes.byref(st),
+                                      N.PROGRESS_FN(), None, None)
+            else:
+                rc = lib.cm_ccg_solve_noalloc(p, ctypes.byref(prm), vp(h), None, vp(u), vp(f), vp(cm),
+                                              ctypes.byref(st), None)
+            res.append((rc, st.nit, u.copy(), f.copy(), cm.copy()))
+        (r0, n0, u0, f0, c0), (r1, n1, u1, f1, c1) = res
+        out.append(dict(grid=[nx, ny], rc=[r0, r1], nit=[n0, n1],
+                        equal=bool(np.array_equal(u0, u1) and np.array_equal(f0, f1) and np.array_equal(c0, c1))))
+    return dict(calls=out)
+
+
 def operators(case):
     nx, ny, bc = case["nx"], case["ny"], case["bc"]
     p, mx, my = plan(nx, ny, bc, 0, 1.0, 2.0, 1.7)
@@ -89,5 +121,5 @@ def operators(case):
 
 if __name__ == "__main__":
     case = json.loads(sys.argv[1])
-    out = {"solve": solve, "ops": operators}[case["kind"]](case)
+    out = {"solve": solve, "ops": operators, "noalloc": noalloc}[case["kind"]](case)
     print(json.dumps(out), flush=True)

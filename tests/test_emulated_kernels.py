@@ -69,3 +69,14 @@ def test_emulated_operators_all_shapes(emu_lib, nx, ny, bc):
     r = run_case(dict(kind="ops", nx=nx, ny=ny, bc=bc))
     assert r["shapes"] >= 4
     assert r["worst"] < 1e-9
+
+
+@pytest.mark.parametrize("bc,prec", [(0, 0), (1, 1)])
+def test_emulated_noalloc_equals_solve(emu_lib, bc, prec):
+    # the allocation-free entry point (workspace in the static scratch) gives
+    # bitwise the results of cm_ccg_solve, on two calls in a row with the
+    # second grid smaller
+    grids = [[32, 32], [24, 16]] if bc == 0 else [[24, 20], [16, 12]]
+    r = run_case(dict(kind="noalloc", bc=bc, prec=prec, grids=grids))
+    for c in r["calls"]:
+        assert c["rc"] == [0, 0] and c["nit"][0] == c["nit"][1] > 0 and c["equal"], c
