@@ -56,7 +56,7 @@ def kernel_bytes(name, M, Mt, precision="fp64"):
         "phase_b": M * (fw + w + gw + 1),              # f, t, gap, mask
         "force_r2c": M * (2 * fw + w + gw + 2 + w),    # f, t, gap, mask -> f, mask, S
         "col_conv2": int(2.5 * w * M),
-        "disp_c2r": M * (w + 2 * gw + fw + 2) + heights,  # S, gap, f, mask -> gap, mask (+ heights)
+        "disp_c2r": M * (w + 2 * gw + 1) + heights,  # S, gap, mask -> gap (+ heights)
     }[name]
 
 
@@ -70,17 +70,17 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_ccg_c2_2048_periodic.
 
 
 def measured_traffic(role, precision="fp64"):
-    """HBM bytes per launch of a kernel role from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE runs (already scaled by the copy-kernel factors);
-    None when no such profile is present."""
+    """HBM bytes per launch of a kernel role ("dir_r2c", "col_conv1", "phase_b",
+    "force_r2c", "col_conv2", "disp_c2r") of the FP64 or FP32 path from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (already scaled by the
+    copy-kernel factors); None when no such profile is present."""
     try:
         with open(TRAFFIC_FILE) as fh:
             d = json.load(fh)
     except (OSError, ValueError):
         return None
-    w = "double" if precision == "fp64" else "float"
-    for name, k in d.get("kernels", {}).items():
-        if k.get("role") == role and f"<{w}" in name:
+    for k in d.get("kernels", {}).values():
+        if k.get("role") == role and k.get("precision") == precision:
             return k["fetch_bytes"] + k["write_bytes"]
     return None
 
@@ -385,7 +385,7 @@ def main():
                    "path": head, "precision_verified_le_1e-6": verified,
                    "timed_region": "resident session, exact --steps iterations, tolerances unreachable"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8000.0, "traffic": measured_traffic(dom.rstrip("12"), head),
+                     "frac": achieved / 8000.0, "traffic": measured_traffic(dom, head),
                      "bytes_per_launch": b, "avg_us": ku[dom]},
         "cpu_baseline": cpu,
         "paths": {p: {"value": rate[p], "ms_per_step": tp[p]["seconds"] / args.steps * 1e3,

@@ -990,8 +990,10 @@ __device__ constexpr int phase_a_op(int i) { return i == 4 ? RMAX : (i == 5 ? RM
 
 // Contact-mask bits: bit 0 = contact set c of the last finished iteration
 // (result.active_set), bit 1 = gap < 0 at the start of that iteration
-// (result.plastic), bit 2 = contact set of the iteration in progress.
-constexpr uint8_t kCur = 1, kGneg = 2, kNext = 4;
+// (result.plastic), bit 2 = contact set of the iteration in progress, bit 3
+// = final force < 0 (written with bit 2 by the force pass for the contact
+// counts of the next iteration).
+constexpr uint8_t kCur = 1, kGneg = 2, kNext = 4, kNeg = 8;
 
 // Phase A of the first iteration (later iterations fuse it into ConsDisp):
 // contact set and gap statistics (:238-273, :312-318 brackets).  It also
@@ -1296,9 +1298,13 @@ struct ProdForce {
     if (comp) {
       const bool nc = (tensile && gap < 0.0) || (flowing && gap > 0.0);
       if (s->do_nc && nc) fo = fo + s->tau * gap;         // :384
-      cmv = (uint8_t)((c ? kCur | kNext : 0) | (gap < 0.0 ? kGneg : 0));
+      // contact set of the next iteration from the final force (:239-249),
+      // bits 2 and 3 for the displacement pass and the next iteration
+      const bool cn0 = fo < 0.0;
+      const bool cn = cn0 && (!s->has_H || fo > -s->H);
+      cmv = (uint8_t)((c ? kCur : 0) | (gap < 0.0 ? kGneg : 0) | (cn ? kNext : 0) | (cn0 ? kNeg : 0));
+      kadd(acc[0], acc[NS], fo);
     }
-    if (comp) kadd(acc[0], acc[NS], fo);
     return fo;
   }
   __device__ void make(const Geom& g, int row, int col, const Rec& r, TT& a, TT& b, double* acc) const {
@@ -1335,13 +1341,17 @@ struct ProdForce {
 template <typename T, typename TT, bool INC>
 struct ConsDisp {
   using F = T;
-  static constexpr int NS = 8;  // maxdu, max_pen | A_contact, A_cg, Sd, Sdd, gmax, gmin (next iteration)
-  // two output pairs per batch: the record by value and the transform's
-  // registers leave no room for four under 128 VGPRs (four waves per SIMD)
-  static constexpr int kBatch = 2;
+  // maxdu, max_pen | A_contact, A_cg, Sd, Sdd, gmax, gmin of the next
+  // iteration (its contact set comes from the mask bits the force pass wrote
+  // from the final forces)
+  static constexpr int NS = 8;
+  // output pairs per batch: two in FP64 (the record by value and the
+  // transform's registers leave no room for four under 128 VGPRs, four waves
+  // per SIMD), four in FP32 (88 VGPRs with two)
+  static constexpr int kBatch = sizeof(T) == 4 ? 4 : 2;
   __device__ static constexpr int op(int i) { return i < 2 ? RMAX : phase_a_op(i - 2); }
   struct Rec {
-    P2<double> w, f;
+    P2<double> w;
     P2<uint8_t> cm;
     HPair h;
   };
@@ -1352,7 +1362,6 @@ struct ConsDisp {
     const size_t p = (size_t)row * g.My + col;
     Rec r;
     r.w = ld2(in.w + p);
-    r.f = ld2(in.fm + p);
     r.cm = ldm(in.cm + p);
     if (INC) {
       const CPair c = ldc(g, in.surf, row, col);
@@ -1371,9 +1380,10 @@ struct ConsDisp {
   };
   __device__ Ctx ctx() const { return Ctx{in.sc->offset, in.sc->H, in.sc->has_H != 0}; }
   // wo: stored value (gap on comp, displacement elsewhere); v: transform
-  // output; wn: the new stored value
-  __device__ uint8_t one(const Ctx& cx, bool comp, uint8_t cmv, double wo, double fv, double hv, TT v, double& wn,
-                         double* acc) const {
+  // output; wn: the new stored value; the next iteration's contact set is
+  // bit 2 of the mask (written by the force pass)
+  __device__ void one(const Ctx& cx, bool comp, uint8_t cmv, double wo, double hv, TT v, double& wn,
+                      double* acc) const {
     if (INC)
       wn = wo + (double)v;
     else
@@ -1382,18 +1392,23 @@ struct ConsDisp {
     if (comp) {
       const double pen = (cmv & kCur) ? (cx.offset - wn) : 0.0;
       acc[1] = red_apply(RMAX, acc[1], pen);
+      // contact counts and shifted gap sums of the next iteration, shift =
+      // this offset (:239-273)
+      const bool cn = (cmv & kNext) != 0;
+      acc[2] += (cmv & kNeg) ? 1.0 : 0.0;
+      acc[3] += cn ? 1.0 : 0.0;
+      const double d = wn - cx.offset;
+      if (cn) acc[4] += d;                  // sum over the contact set only (:269)
+      acc[5] += (cn ? 1.0 : 0.0) * d * d;   // c * g * g: a NaN gap poisons G as in :273
+      acc[6] = red_apply(RMAX, acc[6], wn);
+      acc[7] = red_apply(RMIN, acc[7], wn);
     }
-    // next shift = this offset
-    const bool cn = phase_a_point(cx.has_H, cx.H, cx.offset, fv, wn, comp, acc + 2);
-    return (uint8_t)((cmv & (kCur | kGneg)) | (cn ? kNext : 0));
   }
   __device__ void take(const Geom& g, int row, int col, const Rec& r, TT a, TT b, double* acc, const Ctx& cx) const {
     double w0, w1;
-    const uint8_t m0 = one(cx, r.h.c0, r.cm.a, r.w.a, r.f.a, r.h.h0, a, w0, acc);
-    const uint8_t m1 = one(cx, r.h.c1, r.cm.b, r.w.b, r.f.b, r.h.h1, b, w1, acc);
-    const size_t p = (size_t)row * g.My + col;
-    st2(in.w + p, w0, w1);
-    stm(in.cm + p, m0, m1);
+    one(cx, r.h.c0, r.cm.a, r.w.a, r.h.h0, a, w0, acc);
+    one(cx, r.h.c1, r.cm.b, r.w.b, r.h.h1, b, w1, acc);
+    st2(in.w + (size_t)row * g.My + col, w0, w1);
   }
   __device__ void finalize(double* a) const {
     Scal* s = in.sc;
