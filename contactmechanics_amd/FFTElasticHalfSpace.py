@@ -74,6 +74,7 @@ class PeriodicFFTElasticHalfSpace(ElasticSubstrate):
         self.precision = precision
         self._device = device
         self._plan = None
+        self._plan_bytes = 0
         self._twin = None
 
     # ---- grid properties (reference :308-430)
@@ -166,12 +167,20 @@ class PeriodicFFTElasticHalfSpace(ElasticSubstrate):
             prec = _native.CM_FP32 if self.precision == "fp32" else _native.CM_FP64
             nx, ny = self.nb_grid_pts
             sx, sy = self.physical_sizes
+            used = ctx.mem_used()
             try:
                 self._plan = _native.Plan(ctx, nx, ny, self._bc, prec, sx, sy, self.young,
                                           **self._plan_kwargs())
             except _native.CmError as err:
                 raise self.Error(str(err)) from err
+            self._plan_bytes = ctx.mem_used() - used
         return self._plan
+
+    def plan_bytes(self):
+        """Device bytes the plan holds (Green's functions, twiddles, spectrum
+        buffer); builds the plan if needed."""
+        self.plan  # noqa: B018
+        return self._plan_bytes
 
     @property
     def torch_dtype(self):
@@ -250,16 +259,123 @@ class PeriodicFFTElasticHalfSpace(ElasticSubstrate):
         """Pixel forces due to displacements (reference :591-611)."""
         return self._apply(disp, "force")
 
-    def evaluate(self, disp, pot=True, forces=False):
-        """Elastic energy and/or forces of a displacement field.
+    # ---- k-space helpers (reference :613-885)
+    @property
+    def torch_complex_dtype(self):
+        import torch
+        return torch.complex64 if self.precision == "fp32" else torch.complex128
 
-        Energy ``-sum(f u)/2`` with ``f = evaluate_force(u)`` in real space.
-        """
-        force = self.evaluate_force(disp)
-        energy = None
-        if pot:
-            energy = float(-0.5 * np.sum(np.asarray(force, dtype=float) * np.asarray(disp, dtype=float)))
-        return energy, (force if forces else None)
+    def _grid_tensor(self, arr, what):
+        """arr on the computational grid as a device tensor of the plan dtype."""
+        import torch
+        if tuple(arr.shape) != tuple(self.nb_subdomain_grid_pts):
+            raise self.Error(
+                f"{what} array has a different shape ({tuple(arr.shape)}) than this halfspace's "
+                f"nb_grid_pts ({self.nb_subdomain_grid_pts})")
+        if _is_torch(arr):
+            return arr.to(device=self._device_str(), dtype=self.torch_dtype).contiguous()
+        return torch.from_numpy(np.ascontiguousarray(arr, dtype=np.float64)).to(
+            device=self._device_str(), dtype=self.torch_dtype)
+
+    def _fourier_tensor(self, arr):
+        import torch
+        if tuple(arr.shape) != tuple(self.nb_fourier_grid_pts):
+            raise self.Error(f"fourier array has shape {tuple(arr.shape)}, expected {self.nb_fourier_grid_pts}")
+        if _is_torch(arr):
+            return arr.to(device=self._device_str(), dtype=self.torch_complex_dtype).contiguous()
+        return torch.from_numpy(np.ascontiguousarray(arr, dtype=np.complex128)).to(
+            device=self._device_str(), dtype=self.torch_complex_dtype)
+
+    @staticmethod
+    def _out(t, like):
+        import torch
+        if _is_torch(like):
+            return t
+        return t.to(torch.complex128 if t.is_complex() else torch.float64).cpu().numpy()
+
+    def _fourier(self, field, kind, what):
+        import torch
+        src = self._grid_tensor(field, what)
+        out = torch.empty(self.nb_fourier_grid_pts, dtype=self.torch_complex_dtype, device=self._device_str())
+        self.plan.fourier(src.data_ptr(), out.data_ptr(), kind)
+        return self._out(out, field)
+
+    def fft(self, field):
+        """Unnormalised forward DFT of a real field on the computational grid,
+        in the fourier layout ``(Mx//2+1, My)`` (r2c along x, reference
+        :119-128)."""
+        return self._fourier(field, 0, "field")
+
+    def evaluate_k_disp(self, forces):
+        """k-space displacements ``G DFT(-f) / area_per_pt`` (reference :613-633)."""
+        return self._fourier(forces, 1, "force")
+
+    def evaluate_k_force(self, disp):
+        """k-space forces ``-K DFT(u) area_per_pt`` (reference :635-648)."""
+        return self._fourier(disp, 2, "displacements")
+
+    def evaluate_k_force_k(self, disp_k):
+        """k-space forces from k-space displacements, ``-K u_k area_per_pt``
+        (reference :650-666)."""
+        import torch
+        src = self._fourier_tensor(disp_k)
+        out = torch.empty_like(src)
+        self.plan.k_force_k(src.data_ptr(), out.data_ptr())
+        return self._out(out, disp_k)
+
+    def evaluate_elastic_energy(self, forces, disp):
+        """``0.5 sum(u (-f))`` (reference :676-684), compensated FP64 on the
+        device; fields on the computational or the topography grid."""
+        f, u = self._as_grid(forces), self._as_grid(disp)
+        return -0.5 * self.plan.dot(u.data_ptr(), f.data_ptr())
+
+    def _as_grid(self, arr):
+        """A field on the computational grid (zero-padded from the topography
+        grid if needed) as a device tensor."""
+        import torch
+        if tuple(np.shape(arr)) == tuple(self.nb_subdomain_grid_pts):
+            return self._grid_tensor(arr, "field")
+        nx, ny = self.nb_grid_pts
+        if tuple(np.shape(arr)) != (nx, ny):
+            raise self.Error(f"field has shape {tuple(np.shape(arr))}, expected {self.nb_subdomain_grid_pts} "
+                             f"or {(nx, ny)}")
+        out = torch.zeros(self.nb_subdomain_grid_pts, dtype=self.torch_dtype, device=self._device_str())
+        out[:nx, :ny] = arr if _is_torch(arr) else torch.from_numpy(np.asarray(arr, dtype=np.float64))
+        return out
+
+    def evaluate_scalar_product_k_space(self, ka, kb):
+        """``sum_ij a_ij b_ij`` from the fourier representations of a and b
+        (power theorem over the half spectrum, reference :686-805)."""
+        a, b = self._fourier_tensor(ka), self._fourier_tensor(kb)
+        return self.plan.scalar_product_k(a.data_ptr(), b.data_ptr())
+
+    def evaluate_elastic_energy_k_space(self, kforces, kdisp):
+        """``-0.5 sum u f`` from the fourier representations (reference :807-837)."""
+        return -0.5 * self.evaluate_scalar_product_k_space(kdisp, kforces)
+
+    def evaluate(self, disp, pot=True, forces=False):
+        """Elastic energy and/or forces of a displacement field (reference
+        :839-862): with ``forces`` the energy is the real-space
+        ``0.5 sum u (-f)``, otherwise it is computed in k-space."""
+        force = potential = None
+        if forces:
+            force = self.evaluate_force(disp)
+            if pot:
+                potential = self.evaluate_elastic_energy(force, disp)
+        elif pot:
+            dispk = self.fft(disp)
+            potential = self.evaluate_elastic_energy_k_space(self.evaluate_k_force_k(dispk), dispk)
+        return potential, force
+
+    def evaluate_k(self, disp_k, pot=True, forces=False):
+        """Energy and/or k-space forces of k-space displacements (reference
+        :864-885)."""
+        force_k = potential = None
+        if forces or pot:
+            force_k = self.evaluate_k_force_k(disp_k)
+            if pot:
+                potential = self.evaluate_elastic_energy_k_space(force_k, disp_k)
+        return potential, force_k
 
 
 class FreeFFTElasticHalfSpace(PeriodicFFTElasticHalfSpace):

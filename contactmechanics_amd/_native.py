@@ -97,6 +97,10 @@ SIGNATURES = {
     "cm_plan_shape": ([_VP] + [ctypes.POINTER(ctypes.c_int)] * 4, ctypes.c_int),
     "cm_evaluate_disp": ([_VP, _VP, _VP, _VP], ctypes.c_int),
     "cm_evaluate_force": ([_VP, _VP, _VP, _VP], ctypes.c_int),
+    "cm_fourier": ([_VP, _VP, _VP, ctypes.c_int, _VP], ctypes.c_int),
+    "cm_k_force_k": ([_VP, _VP, _VP, _VP], ctypes.c_int),
+    "cm_scalar_product_k": ([_VP, _VP, _VP, ctypes.POINTER(ctypes.c_double), _VP], ctypes.c_int),
+    "cm_dot": ([_VP, _VP, _VP, ctypes.POINTER(ctypes.c_double), _VP], ctypes.c_int),
     "cm_ccg_solve": ([_VP, ctypes.POINTER(CcgParams), _VP, _VP, _VP, _VP, _VP,
                       ctypes.POINTER(CcgStats), PROGRESS_FN, _VP, _VP], ctypes.c_int),
     "cm_ccg_solve_noalloc": ([_VP, ctypes.POINTER(CcgParams), _VP, _VP, _VP, _VP, _VP,
@@ -266,3 +270,36 @@ class Plan:
             check(self.lib.cm_evaluate_force(self.handle, u_ptr, f_ptr, st), self.ctx.handle)
         finally:
             self.ctx.run_end()
+
+    def fourier(self, src_ptr, out_ptr, kind):
+        """Forward transform into the fourier layout (kind 0 plain, 1 k-disp, 2 k-force)."""
+        st = self.ctx.run_begin()
+        try:
+            check(self.lib.cm_fourier(self.handle, src_ptr, out_ptr, int(kind), st), self.ctx.handle)
+        finally:
+            self.ctx.run_end()
+
+    def k_force_k(self, in_ptr, out_ptr):
+        st = self.ctx.run_begin()
+        try:
+            check(self.lib.cm_k_force_k(self.handle, in_ptr, out_ptr, st), self.ctx.handle)
+        finally:
+            self.ctx.run_end()
+
+    def scalar_product_k(self, a_ptr, b_ptr):
+        out = ctypes.c_double()
+        st = self.ctx.run_begin()
+        try:
+            check(self.lib.cm_scalar_product_k(self.handle, a_ptr, b_ptr, ctypes.byref(out), st), self.ctx.handle)
+        finally:
+            self.ctx.run_end()
+        return out.value
+
+    def dot(self, a_ptr, b_ptr):
+        out = ctypes.c_double()
+        st = self.ctx.run_begin()
+        try:
+            check(self.lib.cm_dot(self.handle, a_ptr, b_ptr, ctypes.byref(out), st), self.ctx.handle)
+        finally:
+            self.ctx.run_end()
+        return out.value

@@ -1533,6 +1533,92 @@ __global__ void k_unpack_green(Geom g, const cplx<double>* S, double* out) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// k-space helpers (FFTElasticHalfSpace.py:613-885) in the reference's fourier
+// layout: (Mx/2 + 1) x My complex, C order, r2c along x (:119-128).
+
+// Spectrum value X[kx, ky] of a real field from the column-blocked half
+// spectrum (ky along the r2c axis, column 0 packing the DC and Nyquist
+// columns); ky > My/2 by Hermitian symmetry X[kx, ky] = conj X[-kx, -ky].
+template <typename T>
+__device__ __forceinline__ cplx<T> spectrum_at(const Geom& g, const cplx<T>* S, int kx, int ky) {
+  bool cj = false;
+  if (ky > g.N2) {
+    ky = g.My - ky;
+    kx = (g.Mx - kx) % g.Mx;
+    cj = true;
+  }
+  cplx<T> v;
+  if (ky > 0 && ky < g.N2) {
+    v = S[((size_t)(ky / g.Cb) * g.Mx + kx) * g.Cb + ky % g.Cb];
+  } else {
+    const int km = (g.Mx - kx) % g.Mx;
+    const cplx<T> q = S[(size_t)kx * g.Cb], qm = S[(size_t)km * g.Cb];
+    v = ky == 0 ? mk(T(0.5) * (q.x + qm.x), T(0.5) * (q.y - qm.y))     // (Q + conj Q(-k)) / 2
+                : mk(T(0.5) * (q.y + qm.y), T(0.5) * (qm.x - q.x));    // (Q - conj Q(-k)) / 2i
+  }
+  return cj ? conj(v) : v;
+}
+
+// out[kx, ky] = scale * W(kx, ky) * X[kx, ky] (W = 1 for kind 0, else the
+// real kernel W, (N2+1) x Mx rows by |ky|).
+template <typename T>
+__global__ void k_unpack_fourier(Geom g, const cplx<T>* S, const T* W, T scale, cplx<T>* out) {
+  const int Rx = g.Mx / 2 + 1;
+  const size_t n = (size_t)Rx * g.My;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    const int kx = (int)(idx / g.My), ky = (int)(idx - (size_t)kx * g.My);
+    cplx<T> v = spectrum_at(g, S, kx, ky);
+    if (W) v = v * (scale * W[(size_t)(ky <= g.N2 ? ky : g.My - ky) * g.Mx + kx]);
+    out[idx] = v;
+  }
+}
+
+// evaluate_k_force_k (:650-666) on the fourier layout: out = scale * K * in.
+template <typename T>
+__global__ void k_kforce_k(Geom g, const T* K, T scale, const cplx<T>* in, cplx<T>* out) {
+  const size_t n = (size_t)(g.Mx / 2 + 1) * g.My;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    const int kx = (int)(idx / g.My), ky = (int)(idx - (size_t)kx * g.My);
+    out[idx] = in[idx] * (scale * K[(size_t)(ky <= g.N2 ? ky : g.My - ky) * g.Mx + kx]);
+  }
+}
+
+// One compensated FP64 sum across the blocks of a kernel, result to *res.
+struct SumRed {
+  static constexpr int NS = 1;
+  __device__ static constexpr int op(int) { return RCSUM; }
+};
+
+// evaluate_scalar_product_k_space (:686-805): sum over the fourier layout of
+// w(kx) Re(conj(a) b) / (Mx My), w = 1 for kx = 0, 2 inside, and for the last
+// row kx = Mx/2 1 if Mx is even (it holds its own mirror) or 2 if odd.
+template <typename T>
+__global__ void __launch_bounds__(256) k_sum_k(Geom g, const cplx<T>* a, const cplx<T>* b, double* part,
+                                               unsigned* ctr, double* res) {
+  const int Rx = g.Mx / 2 + 1;
+  const size_t n = (size_t)Rx * g.My;
+  double acc[2] = {0.0, 0.0};
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    const int kx = (int)(idx / g.My);
+    const double w = kx == 0 ? 1.0 : (kx == Rx - 1 ? (g.Mx % 2 == 0 ? 1.0 : 2.0) : 2.0);
+    const cplx<T> u = a[idx], v = b[idx];
+    kadd(acc[0], acc[1], w * ((double)u.x * (double)v.x + (double)u.y * (double)v.y));
+  }
+  if (reduce_blocks<SumRed>(acc, part, ctr, gridDim.x, blockIdx.x) && threadIdx.x == 0)
+    *res = kvalue(acc[0], acc[1]) / ((double)g.Mx * g.My);
+}
+
+// sum_i a_i b_i over the (Mx, My) grid (evaluate_elastic_energy, :676-684).
+template <typename T>
+__global__ void __launch_bounds__(256) k_dot(size_t n, const T* a, const T* b, double* part, unsigned* ctr,
+                                             double* res) {
+  double acc[2] = {0.0, 0.0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    kadd(acc[0], acc[1], (double)a[i] * (double)b[i]);
+  if (reduce_blocks<SumRed>(acc, part, ctr, gridDim.x, blockIdx.x) && threadIdx.x == 0) *res = kvalue(acc[0], acc[1]);
+}
+
 }  // namespace cm
 
 // ===========================================================================
@@ -1696,6 +1782,10 @@ struct cm_plan {
   void* K64 = nullptr;
   std::vector<double> G_host;  // FP64 copy (scaled) for inspection
   void* S = nullptr;     // spectrum work buffer for plain operator calls
+  // reduction workspace of the k-space helpers: partials, tickets, result
+  double* red_part = nullptr;
+  unsigned* red_ctr = nullptr;
+  double* red_res = nullptr;
   size_t bytes_owned = 0;
   int nblk_elem = 1;
 };
@@ -1908,6 +1998,9 @@ int plan_free(cm_plan* p) {
   dev_free(ctx, p->G, 0);
   dev_free(ctx, p->K, 0);
   dev_free(ctx, p->S, 0);
+  dev_free(ctx, p->red_part, 0);
+  dev_free(ctx, p->red_ctr, 0);
+  dev_free(ctx, p->red_res, 0);
   if (p->prec != CM_FP64) {
     dev_free(ctx, p->ex64.tw_row, 0);
     dev_free(ctx, p->ex64.tw_col, 0);
@@ -2027,6 +2120,8 @@ int upload_kernels(cm_plan* P, const std::vector<double>& Graw, void*& Gd, void*
   return 0;
 }
 
+constexpr int kRedBlocks = 1024;  // blocks of the k-space helper reductions
+
 template <typename T>
 int plan_build(cm_plan* P) {
   cm_ctx* ctx = P->ctx;
@@ -2049,6 +2144,13 @@ int plan_build(cm_plan* P) {
   const size_t Sb = (size_t)P->Mx * (P->My / 2) * sizeof(cplx<T>);
   if ((rc = dev_alloc(ctx, &P->S, Sb))) return rc;
   P->bytes_owned += Sb;
+  // reduction workspace of the k-space helpers (tickets start at zero; the
+  // last block of every reduction resets them)
+  if ((rc = dev_alloc(ctx, (void**)&P->red_part, kRedBlocks * 2 * sizeof(double)))) return rc;
+  if ((rc = dev_alloc(ctx, (void**)&P->red_ctr, kCtrSlot * sizeof(unsigned)))) return rc;
+  if ((rc = dev_alloc(ctx, (void**)&P->red_res, sizeof(double)))) return rc;
+  P->bytes_owned += kRedBlocks * 2 * sizeof(double) + kCtrSlot * sizeof(unsigned) + sizeof(double);
+  HIPCHK(ctx, hipMemset(P->red_ctr, 0, kCtrSlot * sizeof(unsigned)));
   const size_t M = (size_t)P->Mx * P->My;
   // element-wise kernels: blocks stride over rows; few enough blocks that the
   // last block's pass over the partials stays short
@@ -2151,6 +2253,65 @@ int evaluate(cm_plan* P, const void* in, void* out, int useK, void* stream) {
   launch_col<T, 0>(P->ex, ia, 1, useK, 0, st);
   launch_c2r<T, ConsCopy<T>>(P->ex, ia, 1, 0, st);
   return launch_status(P->ctx);
+}
+
+// ---------------------------------------------------------------------------
+// k-space helpers
+
+// Forward transform of a real (Mx, My) field into the fourier layout, times
+// M G (kind 1: G DFT(-f) / area_per_pt, since G holds -1/(area M)) or M K
+// (kind 2: -K DFT(u) area_per_pt) or nothing (kind 0).
+template <typename T>
+int fourier_out(cm_plan* P, const void* field, void* out, int kind, void* stream) {
+  hipStream_t st = stream ? (hipStream_t)stream : P->ctx->stream;
+  InstArr<T> ia{};
+  ia.arr = nullptr;
+  ia.one.src = (const T*)field;
+  ia.one.S = (cplx<T>*)P->S;
+  launch_r2c<T, ProdCopy<T>>(P->ex, ia, 1, 0, st);
+  launch_col<T, 1>(P->ex, ia, 1, 0, 0, st);
+  const T* W = kind == 1 ? (const T*)P->G : (kind == 2 ? (const T*)P->K : nullptr);
+  const T scale = (T)((double)P->Mx * P->My);
+  hipLaunchKernelGGL(k_unpack_fourier<T>, dim3(1024), dim3(256), 0, st, P->ex.g, (const cplx<T>*)P->S, W, scale,
+                     (cplx<T>*)out);
+  return launch_status(P->ctx);
+}
+
+template <typename T>
+int kforce_k(cm_plan* P, const void* in, void* out, void* stream) {
+  hipStream_t st = stream ? (hipStream_t)stream : P->ctx->stream;
+  const T scale = (T)((double)P->Mx * P->My);
+  hipLaunchKernelGGL(k_kforce_k<T>, dim3(1024), dim3(256), 0, st, P->ex.g, (const T*)P->K, scale,
+                     (const cplx<T>*)in, (cplx<T>*)out);
+  return launch_status(P->ctx);
+}
+
+int reduction_blocks(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>(kRedBlocks, (n + 255) / 256)); }
+
+template <typename T>
+int scalar_product_k(cm_plan* P, const void* a, const void* b, double* out, void* stream) {
+  hipStream_t st = stream ? (hipStream_t)stream : P->ctx->stream;
+  const size_t n = (size_t)(P->Mx / 2 + 1) * P->My;
+  hipLaunchKernelGGL(k_sum_k<T>, dim3(reduction_blocks(n)), dim3(256), 0, st, P->ex.g, (const cplx<T>*)a,
+                     (const cplx<T>*)b, P->red_part, P->red_ctr, P->red_res);
+  int rc = launch_status(P->ctx);
+  if (rc) return rc;
+  HIPCHK(P->ctx, hipMemcpyAsync(out, P->red_res, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(P->ctx, hipStreamSynchronize(st));
+  return 0;
+}
+
+template <typename T>
+int dot(cm_plan* P, const void* a, const void* b, double* out, void* stream) {
+  hipStream_t st = stream ? (hipStream_t)stream : P->ctx->stream;
+  const size_t n = (size_t)P->Mx * P->My;
+  hipLaunchKernelGGL(k_dot<T>, dim3(reduction_blocks(n)), dim3(256), 0, st, n, (const T*)a, (const T*)b,
+                     P->red_part, P->red_ctr, P->red_res);
+  int rc = launch_status(P->ctx);
+  if (rc) return rc;
+  HIPCHK(P->ctx, hipMemcpyAsync(out, P->red_res, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(P->ctx, hipStreamSynchronize(st));
+  return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -2841,6 +3002,30 @@ int cm_evaluate_disp(cm_plan* plan, const void* f, void* u, void* stream) {
 int cm_evaluate_force(cm_plan* plan, const void* u, void* f, void* stream) {
   if (!plan || !f || !u) return fail(plan ? plan->ctx : nullptr, CM_E_ARG, "null argument");
   return plan->prec == CM_FP64 ? evaluate<double>(plan, u, f, 1, stream) : evaluate<float>(plan, u, f, 1, stream);
+}
+
+int cm_fourier(cm_plan* plan, const void* field, void* out_k, int kind, void* stream) {
+  if (!plan || !field || !out_k) return fail(plan ? plan->ctx : nullptr, CM_E_ARG, "null argument");
+  if (kind < 0 || kind > 2) return fail(plan->ctx, CM_E_ARG, "kind must be 0, 1 or 2");
+  return plan->prec == CM_FP64 ? fourier_out<double>(plan, field, out_k, kind, stream)
+                               : fourier_out<float>(plan, field, out_k, kind, stream);
+}
+
+int cm_k_force_k(cm_plan* plan, const void* disp_k, void* out_k, void* stream) {
+  if (!plan || !disp_k || !out_k) return fail(plan ? plan->ctx : nullptr, CM_E_ARG, "null argument");
+  return plan->prec == CM_FP64 ? kforce_k<double>(plan, disp_k, out_k, stream)
+                               : kforce_k<float>(plan, disp_k, out_k, stream);
+}
+
+int cm_scalar_product_k(cm_plan* plan, const void* ka, const void* kb, double* out, void* stream) {
+  if (!plan || !ka || !kb || !out) return fail(plan ? plan->ctx : nullptr, CM_E_ARG, "null argument");
+  return plan->prec == CM_FP64 ? scalar_product_k<double>(plan, ka, kb, out, stream)
+                               : scalar_product_k<float>(plan, ka, kb, out, stream);
+}
+
+int cm_dot(cm_plan* plan, const void* a, const void* b, double* out, void* stream) {
+  if (!plan || !a || !b || !out) return fail(plan ? plan->ctx : nullptr, CM_E_ARG, "null argument");
+  return plan->prec == CM_FP64 ? dot<double>(plan, a, b, out, stream) : dot<float>(plan, a, b, out, stream);
 }
 
 int cm_ccg_solve(cm_plan* plan, const cm_ccg_params* params, const double* heights, const uint8_t* surf_mask,

@@ -144,6 +144,26 @@ int cm_evaluate_disp(cm_plan* plan, const void* f, void* u, void* stream);
 /* f = evaluate_force(u) (FFTElasticHalfSpace.py:591-611). */
 int cm_evaluate_force(cm_plan* plan, const void* u, void* f, void* stream);
 
+/* ---- k-space helpers (FFTElasticHalfSpace.py:613-885) -----------------
+ * Fourier fields use the reference's layout: (Mx/2 + 1) x My complex, C
+ * order, the real-to-complex axis along x (FFTElasticHalfSpace.py:119-128),
+ * unnormalised forward transform; complex values are (re, im) pairs in the
+ * plan precision.  The helpers share the plan's spectrum buffer and
+ * reduction workspace: calls on one plan must not overlap. */
+/* Forward transform of a real (Mx, My) field: kind 0 the plain DFT (the
+ * fourier buffer of fftengine.fft); kind 1 evaluate_k_disp (:613-633),
+ * G DFT(-f) / area_per_pt; kind 2 evaluate_k_force (:635-648),
+ * -K DFT(u) area_per_pt. */
+int cm_fourier(cm_plan* plan, const void* field, void* out_k, int kind, void* stream);
+/* evaluate_k_force_k (:650-666): out = -K disp_k area_per_pt (in place allowed). */
+int cm_k_force_k(cm_plan* plan, const void* disp_k, void* out_k, void* stream);
+/* evaluate_scalar_product_k_space (:686-805): the Hermitian-weighted sum of
+ * Re(conj(ka) kb) / (Mx My), compensated FP64, synchronously to host *out. */
+int cm_scalar_product_k(cm_plan* plan, const void* ka, const void* kb, double* out, void* stream);
+/* sum of a b over the (Mx, My) grid, compensated FP64, to host *out
+ * (evaluate_elastic_energy, :676-684, is 0.5 * dot(u, -f)). */
+int cm_dot(cm_plan* plan, const void* a, const void* b, double* out, void* stream);
+
 /* Constrained conjugate gradients (ConstrainedConjugateGradients.py:47-509).
  *  heights   device FP64 (nx, ny)
  *  surf_mask device uint8 (nx, ny), 1 = valid pixel, NULL = no mask

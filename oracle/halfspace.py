@@ -221,6 +221,76 @@ class PeriodicHalfSpace:
     def check(self, force=None):
         pass
 
+    # ---- k-space helpers (FFTElasticHalfSpace.py:613-885), fourier layout
+    # (Mx//2+1, My) with the r2c axis along x, unnormalised forward DFT
+
+    @property
+    def nb_fourier_grid_pts(self):
+        mx, my = self.nb_domain_grid_pts
+        return (mx // 2 + 1, my)
+
+    def fft(self, field):
+        return _rfft2_axis0(np.asarray(field, dtype=np.float64), np.float64)
+
+    def evaluate_k_disp(self, forces):
+        """``G DFT(-f) / area_per_pt`` (:613-633)."""
+        if forces.shape != self.nb_subdomain_grid_pts:
+            raise SubstrateError("force array has a different shape")
+        return self.greens_function * self.fft(-forces) / self.area_per_pt
+
+    def evaluate_k_force(self, disp):
+        """``-K DFT(u) area_per_pt`` (:635-648)."""
+        if disp.shape != self.nb_subdomain_grid_pts:
+            raise SubstrateError("displacements array has a different shape")
+        return -self.surface_stiffness * self.fft(disp) * self.area_per_pt
+
+    def evaluate_k_force_k(self, disp_k):
+        """``-K u_k area_per_pt`` (:650-666)."""
+        return -self.surface_stiffness * disp_k * self.area_per_pt
+
+    def evaluate_elastic_energy(self, forces, disp):
+        """``0.5 u . (-f)`` (:676-684)."""
+        return 0.5 * np.dot(np.ravel(disp), np.ravel(-forces))
+
+    def evaluate_scalar_product_k_space(self, ka, kb):
+        """Power theorem over the half spectrum (:686-805, serial case): the
+        rows kx of the r2c axis that stand for two spectrum rows (kx and -kx)
+        count twice; kx = 0 once, and the last row kx = Mx//2 once when Mx is
+        even (it is its own mirror) and twice when Mx is odd."""
+        mx = self.nb_domain_grid_pts[0]
+        rows = ka.shape[0]
+        fact0 = 1
+        factend = 0 if rows == 1 else (2 if mx % 2 == 1 else 1)
+        factmiddle = 2 if rows > 2 else 0
+        s = (factmiddle * np.vdot(ka[1:-1], kb[1:-1]).real + fact0 * np.vdot(ka[0], kb[0]).real
+             + factend * np.vdot(ka[-1], kb[-1]).real)
+        return s / np.prod(self.nb_domain_grid_pts)
+
+    def evaluate_elastic_energy_k_space(self, kforces, kdisp):
+        """``-0.5 sum u f`` from fourier representations (:807-837)."""
+        return -0.5 * self.evaluate_scalar_product_k_space(kdisp, kforces)
+
+    def evaluate(self, disp, pot=True, forces=False):
+        """(:839-862) real-space energy with forces, k-space energy without."""
+        force = potential = None
+        if forces:
+            force = self.evaluate_force(disp)
+            if pot:
+                potential = self.evaluate_elastic_energy(force, disp)
+        elif pot:
+            dispk = self.fft(disp)
+            potential = self.evaluate_elastic_energy_k_space(self.evaluate_k_force_k(dispk), dispk)
+        return potential, force
+
+    def evaluate_k(self, disp_k, pot=True, forces=False):
+        """(:864-885)."""
+        force_k = potential = None
+        if forces or pot:
+            force_k = self.evaluate_k_force_k(disp_k)
+            if pot:
+                potential = self.evaluate_elastic_energy_k_space(force_k, disp_k)
+        return potential, force_k
+
 
 class FreeHalfSpace(PeriodicHalfSpace):
     """Oracle ``FreeFFTElasticHalfSpace`` (Hockney 2N padding).

@@ -99,6 +99,34 @@ def noalloc(case):
     return dict(calls=out)
 
 
+def kspace(case):
+    """k-space helpers of the C ABI against the oracle (FP64)."""
+    nx, ny, bc = case["nx"], case["ny"], case["bc"]
+    p, mx, my = plan(nx, ny, bc, 0, 1.3, 2.1, 2.2)
+    c = (FreeHalfSpace if bc else PeriodicHalfSpace)((nx, ny), 2.2, (1.3, 2.1))
+    rng = np.random.default_rng(nx + 7 * ny)
+    f = np.ascontiguousarray(rng.standard_normal((mx, my)))
+    u = np.ascontiguousarray(rng.standard_normal((mx, my)))
+    vp = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
+    errs = {}
+    for kind, ref in ((0, c.fft(f)), (1, c.evaluate_k_disp(f)), (2, c.evaluate_k_force(f))):
+        out = np.zeros((mx // 2 + 1, my), np.complex128)
+        assert lib.cm_fourier(p, vp(f), vp(out), kind, None) == 0
+        errs[f"fourier{kind}"] = float(np.max(np.abs(out - ref)) / np.max(np.abs(ref)))
+    ku = np.ascontiguousarray(c.fft(u))
+    out = np.zeros_like(ku)
+    assert lib.cm_k_force_k(p, vp(ku), vp(out), None) == 0
+    ref = c.evaluate_k_force_k(ku)
+    errs["k_force_k"] = float(np.max(np.abs(out - ref)) / np.max(np.abs(ref)))
+    kf = np.ascontiguousarray(c.fft(f))
+    r = ctypes.c_double()
+    assert lib.cm_scalar_product_k(p, vp(kf), vp(ku), ctypes.byref(r), None) == 0
+    errs["scalar_product"] = abs(r.value - float(np.sum(f * u))) / float(np.sum(np.abs(f * u)))
+    assert lib.cm_dot(p, vp(f), vp(u), ctypes.byref(r), None) == 0
+    errs["dot"] = abs(r.value - float(np.sum(f * u))) / float(np.sum(np.abs(f * u)))
+    return errs
+
+
 def operators(case):
     nx, ny, bc = case["nx"], case["ny"], case["bc"]
     p, mx, my = plan(nx, ny, bc, 0, 1.0, 2.0, 1.7)
@@ -121,5 +149,5 @@ def operators(case):
 
 if __name__ == "__main__":
     case = json.loads(sys.argv[1])
-    out = {"solve": solve, "ops": operators, "noalloc": noalloc}[case["kind"]](case)
+    out = {"solve": solve, "ops": operators, "noalloc": noalloc, "kspace": kspace}[case["kind"]](case)
     print(json.dumps(out), flush=True)

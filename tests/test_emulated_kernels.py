@@ -80,3 +80,13 @@ def test_emulated_noalloc_equals_solve(emu_lib, bc, prec):
     r = run_case(dict(kind="noalloc", bc=bc, prec=prec, grids=grids))
     for c in r["calls"]:
         assert c["rc"] == [0, 0] and c["nit"][0] == c["nit"][1] > 0 and c["equal"], c
+
+
+@pytest.mark.parametrize("nx,ny,bc", [(12, 10, 0), (15, 8, 0), (6, 5, 1)])
+def test_emulated_kspace_helpers(emu_lib, nx, ny, bc):
+    # the fourier-layout unpack (incl. odd Mx and the packed DC/Nyquist
+    # column), k-space force and the compensated reductions, against the oracle
+    r = run_case(dict(kind="kspace", nx=nx, ny=ny, bc=bc))
+    tol = 1e-13 if bc == 0 else 1e-12
+    for k, v in r.items():
+        assert v < tol, (k, v)

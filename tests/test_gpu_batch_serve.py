@@ -135,3 +135,27 @@ def test_session_restores_displacements(gpu, bc):
     expect[:n, :n] = np.maximum(0.0, p["heights"])
     assert np.array_equal(u0, expect)
     s0.close()
+
+
+def test_server_cancel_and_memory_release(gpu):
+    """A waiting request that is cancelled never runs; an active one ends
+    after its chunk in flight and releases its memory; the others finish as
+    if alone.  After the run only the shared plans stay counted."""
+    big = problem(512, "periodic", 300, gtol=1e-12)
+    big.update(index=0, arrival=0.0, maxiter=200000)
+    small = problem(64, "periodic", 301, gtol=1e-6)
+    small.update(index=1, arrival=0.0)
+    late = problem(96, "free", 302, gtol=1e-6)
+    late.update(index=2, arrival=0.05)
+    srv = ContactServer(precision="fp32", streams=2)
+    srv.cancel(2)  # dropped when it arrives
+
+    def on_done(rec, s):
+        if rec["index"] == 1:
+            srv.cancel(0)  # the long request is still running
+
+    done, summary = srv.run([big, small, late], on_done=on_done)
+    status = {d["index"]: d["status"] for d in done}
+    assert status == {0: "cancelled", 1: 0, 2: "cancelled"}, status
+    assert summary["requests"] == 1 and summary["cancelled"] == 2
+    assert srv.used == sum(srv.plan_bytes.values())
