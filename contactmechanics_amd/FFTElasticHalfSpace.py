@@ -24,6 +24,7 @@ forces, see ``Optimization/ConstrainedConjugateGradients.py``).
 import numpy as np
 
 from . import _native
+from .gridmap import GridMap
 from .Substrates import ElasticSubstrate
 
 
@@ -44,6 +45,7 @@ class PeriodicFFTElasticHalfSpace(ElasticSubstrate):
     name = "periodic_fft_elastic_halfspace"
     _periodic = True
     _bc = _native.CM_PERIODIC
+    periodicity = (True, True)
 
     def __init__(self, nb_grid_pts, young, physical_sizes=2 * np.pi, stiffness_q0=None,
                  thickness=None, poisson=0.0, superclass=True, fft="serial", communicator=None,
@@ -76,6 +78,7 @@ class PeriodicFFTElasticHalfSpace(ElasticSubstrate):
         self._plan = None
         self._plan_bytes = 0
         self._twin = None
+        self._gmap = None
 
     # ---- grid properties (reference :308-430)
     @property
@@ -160,17 +163,25 @@ class PeriodicFFTElasticHalfSpace(ElasticSubstrate):
         return dict(poisson=self.poisson, q0_mode=q0_mode, stiffness_q0=q0, thickness=self.thickness)
 
     @property
+    def gridmap(self):
+        """How the computational grid maps to the plan's (identity unless My
+        is odd; see ``gridmap.py``)."""
+        if self._gmap is None:
+            self._gmap = GridMap(*self.nb_domain_grid_pts, *self.periodicity)
+        return self._gmap
+
+    @property
     def plan(self):
         """The device plan (built on first use: needs a HIP device)."""
         if self._plan is None:
             ctx = _native.Context.get(self._device)
             prec = _native.CM_FP32 if self.precision == "fp32" else _native.CM_FP64
-            nx, ny = self.nb_grid_pts
-            sx, sy = self.physical_sizes
+            gm = self.gridmap
+            nx, ny, sx, sy, px, py = gm.plan_args(*self.nb_grid_pts, *self.physical_sizes, *self.periodicity)
             used = ctx.mem_used()
             try:
-                self._plan = _native.Plan(ctx, nx, ny, self._bc, prec, sx, sy, self.young,
-                                          **self._plan_kwargs())
+                self._plan = _native.Plan(ctx, nx, ny, self._bc, prec, sx, sy, self.young, periodic_x=px,
+                                          periodic_y=py, tile_y=gm.tile, **self._plan_kwargs())
             except _native.CmError as err:
                 raise self.Error(str(err)) from err
             self._plan_bytes = ctx.mem_used() - used
@@ -195,6 +206,7 @@ class PeriodicFFTElasticHalfSpace(ElasticSubstrate):
             twin.precision = "fp32"
             twin._plan = None
             twin._twin = None
+            twin._gmap = None
             self._twin = twin
         return self._twin
 
@@ -206,11 +218,15 @@ class PeriodicFFTElasticHalfSpace(ElasticSubstrate):
     def _kernel_ref_layout(self):
         """Green's function (unscaled, real) in the reference's half-spectrum
         layout ``(Mx//2+1, My)``: r2c along axis 0 (reference :119-128)."""
-        mx, my = self.nb_domain_grid_pts
+        mx, my = self.gridmap.plan_shape
         g = self.plan.green_host()  # (my//2+1, mx), includes -1/(area*M)
         g = g / (-1.0 / (self.area_per_pt * mx * my))
         j = np.arange(my)
-        return g[np.minimum(j, my - j)][:, : mx // 2 + 1].T.copy()
+        g = g[np.minimum(j, my - j)][:, : mx // 2 + 1].T.copy()
+        if self.gridmap.identity:
+            return g
+        import torch
+        return self.gridmap.green_inv(torch.from_numpy(g)).numpy()
 
     @property
     def greens_function(self):
@@ -236,17 +252,19 @@ class PeriodicFFTElasticHalfSpace(ElasticSubstrate):
         dev = self._device_str()
         if _is_torch(arr):
             src = arr.to(device=dev, dtype=self.torch_dtype).contiguous()
-            out = torch.empty_like(src)
             ret_torch = True
         else:
             src = torch.from_numpy(np.ascontiguousarray(arr, dtype=np.float64)).to(
                 device=dev, dtype=self.torch_dtype)
-            out = torch.empty_like(src)
             ret_torch = False
+        gm = self.gridmap
+        src = gm.fwd(src)
+        out = torch.empty_like(src)
         if which == "disp":
             plan.evaluate_disp(src.data_ptr(), out.data_ptr())
         else:
             plan.evaluate_force(src.data_ptr(), out.data_ptr())
+        out = gm.inv(out)
         if ret_torch:
             return out
         return out.to(dtype=torch.float64).cpu().numpy()
@@ -295,10 +313,12 @@ class PeriodicFFTElasticHalfSpace(ElasticSubstrate):
 
     def _fourier(self, field, kind, what):
         import torch
-        src = self._grid_tensor(field, what)
-        out = torch.empty(self.nb_fourier_grid_pts, dtype=self.torch_complex_dtype, device=self._device_str())
+        gm = self.gridmap
+        src = gm.fwd(self._grid_tensor(field, what))
+        pmx, pmy = gm.plan_shape
+        out = torch.empty((pmx // 2 + 1, pmy), dtype=self.torch_complex_dtype, device=self._device_str())
         self.plan.fourier(src.data_ptr(), out.data_ptr(), kind)
-        return self._out(out, field)
+        return self._out(gm.fourier_inv(out), field)
 
     def fft(self, field):
         """Unnormalised forward DFT of a real field on the computational grid,
@@ -318,16 +338,18 @@ class PeriodicFFTElasticHalfSpace(ElasticSubstrate):
         """k-space forces from k-space displacements, ``-K u_k area_per_pt``
         (reference :650-666)."""
         import torch
-        src = self._fourier_tensor(disp_k)
+        gm = self.gridmap
+        src = gm.fourier_fwd(self._fourier_tensor(disp_k))
         out = torch.empty_like(src)
         self.plan.k_force_k(src.data_ptr(), out.data_ptr())
-        return self._out(out, disp_k)
+        return self._out(gm.fourier_inv(out), disp_k)
 
     def evaluate_elastic_energy(self, forces, disp):
         """``0.5 sum(u (-f))`` (reference :676-684), compensated FP64 on the
         device; fields on the computational or the topography grid."""
-        f, u = self._as_grid(forces), self._as_grid(disp)
-        return -0.5 * self.plan.dot(u.data_ptr(), f.data_ptr())
+        gm = self.gridmap
+        f, u = gm.fwd(self._as_grid(forces)), gm.fwd(self._as_grid(disp))
+        return -0.5 * gm.sum_inv(self.plan.dot(u.data_ptr(), f.data_ptr()))
 
     def _as_grid(self, arr):
         """A field on the computational grid (zero-padded from the topography
@@ -346,8 +368,9 @@ class PeriodicFFTElasticHalfSpace(ElasticSubstrate):
     def evaluate_scalar_product_k_space(self, ka, kb):
         """``sum_ij a_ij b_ij`` from the fourier representations of a and b
         (power theorem over the half spectrum, reference :686-805)."""
-        a, b = self._fourier_tensor(ka), self._fourier_tensor(kb)
-        return self.plan.scalar_product_k(a.data_ptr(), b.data_ptr())
+        gm = self.gridmap
+        a, b = gm.fourier_fwd(self._fourier_tensor(ka)), gm.fourier_fwd(self._fourier_tensor(kb))
+        return gm.sum_inv(self.plan.scalar_product_k(a.data_ptr(), b.data_ptr()))
 
     def evaluate_elastic_energy_k_space(self, kforces, kdisp):
         """``-0.5 sum u f`` from the fourier representations (reference :807-837)."""
@@ -384,6 +407,7 @@ class FreeFFTElasticHalfSpace(PeriodicFFTElasticHalfSpace):
     name = "free_fft_elastic_halfspace"
     _periodic = False
     _bc = _native.CM_FREE
+    periodicity = (False, False)
 
     class FreeBoundaryError(Exception):
         """Forces overlap into the padding region (reference :1207-1215)."""
@@ -486,3 +510,72 @@ class FreeFFTElasticHalfSpace(PeriodicFFTElasticHalfSpace):
     def check(self, force=None):
         if self._check_boundaries:
             self.check_boundaries(force)
+
+
+class SemiPeriodicFFTElasticHalfSpace(FreeFFTElasticHalfSpace):
+    """Half-space periodic along some directions and free along the others
+    (reference ``FFTElasticHalfSpace.py:1286-1615``).
+
+    The computational grid has ``n`` points along a periodic direction and
+    ``2n - 1`` (Hockney padding) along a free one (reference :1344-1352).  The
+    Green's function is the Love kernel at numpy-fftfreq-wrapped coordinates
+    summed over ``n_images`` periodic images per side (reference
+    :1354-1497), computed and transformed on the device (``CM_SEMI``).  Odd
+    computational sides are handled by the grid map (``gridmap.py``); with no
+    periodic direction and both sides odd, the operators work and the solver
+    refuses the grid.
+    """
+
+    name = "semi_periodic_fft_elastic_halfspace"
+    _bc = _native.CM_SEMI
+
+    def __init__(self, nb_grid_pts, young, physical_sizes=(2 * np.pi, 2 * np.pi), periodicity=(False, True),
+                 n_images=10, fft="serial", communicator=None, check_boundaries=False, fftengine=None,
+                 precision="fp64", device=None):
+        super().__init__(nb_grid_pts, young, physical_sizes, fft=fft, communicator=communicator,
+                         check_boundaries=check_boundaries, precision=precision, device=device)
+        self.periodicity = tuple(bool(p) for p in periodicity)
+        self.n_images = int(n_images)
+        self._comp_nb_grid_pts = tuple(n if p else 2 * n - 1 for n, p in zip(self.nb_grid_pts, self.periodicity))
+
+    def _plan_kwargs(self):
+        return dict(poisson=0.0, q0_mode=0, stiffness_q0=0.0, thickness=None, n_images=self.n_images)
+
+    def spawn_child(self, nb_grid_pts):
+        size = tuple(nb_grid_pts[i] / float(self.nb_grid_pts[i]) * self.physical_sizes[i]
+                     for i in range(self.dim))
+        return type(self)(nb_grid_pts, self.young, size, periodicity=self.periodicity, n_images=self.n_images,
+                          precision=self.precision, device=self._device)
+
+    def evaluate_force(self, disp):
+        """Forces from displacements; a smaller displacement array is
+        zero-padded to the computational grid first (reference :1522-1542)."""
+        if tuple(disp.shape) != tuple(self.nb_subdomain_grid_pts):
+            if _is_torch(disp):
+                import torch
+                padded = torch.zeros(self.nb_subdomain_grid_pts, dtype=self.torch_dtype, device=self._device_str())
+            else:
+                padded = np.zeros(self.nb_subdomain_grid_pts)
+            padded[:disp.shape[0], :disp.shape[1]] = disp
+            disp = padded
+        return super().evaluate_force(disp)
+
+    @property
+    def G_real(self):
+        """The real-space kernel (sum over images) on the computational grid,
+        from the device Green's function (reference :1379-1381 keeps it)."""
+        mx, my = self.nb_domain_grid_pts
+        return np.fft.irfft2(self.greens_function, s=(my, mx), axes=(1, 0))
+
+    def get_G_real(self):
+        """The real-space kernel with G(0, 0) at the array centre (reference
+        :1544-1579)."""
+        mx, my = self.nb_domain_grid_pts
+        m = np.round(np.fft.fftfreq(mx) * mx).astype(int)
+        n = np.round(np.fft.fftfreq(my) * my).astype(int)
+        return self.G_real[np.ix_(np.argsort(m), np.argsort(n))]
+
+    def get_G_real_slices(self):
+        """Middle slices of ``get_G_real`` along x and y (reference :1581-1604)."""
+        g = self.get_G_real()
+        return g[:, g.shape[1] // 2], g[g.shape[0] // 2, :]

@@ -32,11 +32,13 @@ FP32 forces (``initial_forces``) to the requested tolerances.  The result has
 """
 
 import ctypes
+import types
 
 import numpy as np
 import scipy.optimize as optim
 
 from .. import _native
+from ..session import _EXTENSIVE
 from ..Tools.Logger import quiet
 
 
@@ -89,6 +91,19 @@ def default_pentol(topography, heights, nb_grid_pts, offset):
     if pentol == 0:
         pentol = 1e-3
     return pentol
+
+
+
+
+def _caller_stats(st, tile):
+    """The solve statistics for the caller's grid (a plan holding two periods
+    of it counts the grid sums twice)."""
+    if tile == 1:
+        return st
+    d = st.as_dict()
+    for k in _EXTENSIVE:
+        d[k] = d[k] / tile
+    return types.SimpleNamespace(**d)
 
 
 def _to_device_field(arr, substrate, name):
@@ -154,11 +169,19 @@ def constrained_conjugate_gradients(substrate, topography, hardness=None, extern
 
     plan = substrate.plan
     dev = substrate._device_str()
+    # the plan's grid: the caller's unless My is odd (transposed and/or two
+    # periods along a periodic y, see gridmap.py); every field is mapped in
+    # and out, and grid sums (load, areas, energy) scale with the periods
+    gm = substrate.gridmap
+    if not gm.solvable:
+        raise substrate.Error(f"no solve on the computational grid {substrate.nb_domain_grid_pts} with periodicity "
+                              f"{substrate.periodicity}: both sides odd and no periodic direction")
+    tile = gm.tile
     M_shape = tuple(substrate.nb_subdomain_grid_pts)
-    h_dev = h_np.to(dev) if _is_torch(h_np) else torch.from_numpy(h_np).to(dev)
+    h_dev = gm.fwd(h_np.to(dev) if _is_torch(h_np) else torch.from_numpy(h_np).to(dev))
     m_dev = None
     if valid is not None:
-        m_dev = torch.from_numpy(np.ascontiguousarray(valid, dtype=np.uint8)).to(dev)
+        m_dev = gm.fwd(torch.from_numpy(np.ascontiguousarray(valid, dtype=np.uint8)).to(dev))
     if initial_displacements is None:
         u = torch.zeros(M_shape, dtype=substrate.torch_dtype, device=dev)
     else:
@@ -169,12 +192,13 @@ def constrained_conjugate_gradients(substrate, topography, hardness=None, extern
     else:
         f = _to_device_field(initial_forces, substrate, "initial_forces")
         init_mode = 1
-    cmask = torch.zeros(M_shape, dtype=torch.uint8, device=dev)
+    u, f = gm.fwd(u), gm.fwd(f)
+    cmask = torch.zeros(gm.plan_shape, dtype=torch.uint8, device=dev)
 
     per_iteration = callback is not None or getattr(logger, "active", False)
     params = _native.CcgParams(
         load_control=1 if external_force is not None else 0,
-        external_force=float(external_force) if external_force is not None else 0.0,
+        external_force=float(external_force) * tile if external_force is not None else 0.0,
         offset=float(offset) if offset is not None else float("nan"),
         has_hardness=1 if hardness is not None else 0,
         hardness=float(hardness) if hardness is not None else 0.0,
@@ -195,7 +219,7 @@ def constrained_conjugate_gradients(substrate, topography, hardness=None, extern
         # including the maxiter and NaN iterations; the converged one is
         # reported with the result
         try:
-            st = st_ptr.contents
+            st = _caller_stats(st_ptr.contents, tile)
             if st.nit == 0 or st.status == _native.CM_CONVERGED:
                 return
             headers, values = _log_row(st, hardness, verbose, n_surf)
@@ -206,7 +230,7 @@ def constrained_conjugate_gradients(substrate, topography, hardness=None, extern
                          rms_penetration=float(st.rms_pen), max_penetration=float(st.max_pen),
                          max_pressure=float(st.max_pres), pad_pressure=float(st.pad_pres),
                          penetration_tol=float(pentol), pressure_tol=float(forcetol))
-                callback(st.nit + it_offset, f.to(torch.float64).cpu().numpy(), d)
+                callback(st.nit + it_offset, gm.inv(f).to(torch.float64).cpu().numpy(), d)
         except Exception as err:  # re-raised after the solve returns
             errors.append(err)
 
@@ -228,8 +252,8 @@ def constrained_conjugate_gradients(substrate, topography, hardness=None, extern
     if rc < 0:
         _native.check(rc, plan.ctx.handle)
 
-    return _assemble_result(substrate, stats, u, f, cmask, hardness, return_device,
-                            logger, n_surf, maxiter, verbose)
+    return _assemble_result(substrate, _caller_stats(stats, tile), gm.inv(u), gm.inv(f), gm.inv(cmask), hardness,
+                            return_device, logger, n_surf, maxiter, verbose)
 
 
 def _log_row(st, hardness, verbose, n_surf):

@@ -1,11 +1,13 @@
 """MI355X (gfx950) contact solver with the ContactMechanics hot-path API.
 
-Polonsky-Keer constrained conjugate gradients on periodic and free (2N
-zero-padded) FFT elastic half-spaces, with the plastic hardness cap, running
+Polonsky-Keer constrained conjugate gradients on periodic, free (2N
+zero-padded) and semi-periodic FFT elastic half-spaces, with the plastic
+hardness cap, running
 in hand-written HIP kernels behind the C ABI ``include/cmhip.h``.
 """
 
-from .FFTElasticHalfSpace import FreeFFTElasticHalfSpace, PeriodicFFTElasticHalfSpace  # noqa: F401
+from .FFTElasticHalfSpace import (FreeFFTElasticHalfSpace, PeriodicFFTElasticHalfSpace,  # noqa: F401
+                                  SemiPeriodicFFTElasticHalfSpace)
 from .Factory import make_plastic_system, make_system  # noqa: F401
 from .Optimization import constrained_conjugate_gradients  # noqa: F401
 from .PlasticSystemSpecialisations import PlasticNonSmoothContactSystem  # noqa: F401

@@ -29,6 +29,7 @@ CM_E_UNSUPPORTED = -6
 CM_E_BISECT = -7
 CM_PERIODIC = 0
 CM_FREE = 1
+CM_SEMI = 2
 CM_FP64 = 0
 CM_FP32 = 1
 
@@ -49,7 +50,9 @@ class SubstrateDesc(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("bc", ctypes.c_int),
                 ("precision", ctypes.c_int), ("sx", ctypes.c_double), ("sy", ctypes.c_double),
                 ("young", ctypes.c_double), ("poisson", ctypes.c_double), ("q0_mode", ctypes.c_int),
-                ("stiffness_q0", ctypes.c_double), ("thickness", ctypes.c_double)]
+                ("stiffness_q0", ctypes.c_double), ("thickness", ctypes.c_double),
+                ("periodic_x", ctypes.c_int), ("periodic_y", ctypes.c_int), ("n_images", ctypes.c_int),
+                ("tile_y", ctypes.c_int)]
 
 
 class CcgParams(ctypes.Structure):
@@ -212,13 +215,16 @@ class Plan:
     """Owns a ``cm_plan`` (Green's function, twiddles, launch shapes)."""
 
     def __init__(self, ctx, nx, ny, bc, precision, sx, sy, young, poisson=0.0,
-                 q0_mode=0, stiffness_q0=0.0, thickness=None):
+                 q0_mode=0, stiffness_q0=0.0, thickness=None, periodic_x=0, periodic_y=1, n_images=10,
+                 tile_y=1):
         self.ctx = ctx
         self.lib = ctx.lib
         d = SubstrateDesc(nx=int(nx), ny=int(ny), bc=int(bc), precision=int(precision),
                           sx=float(sx), sy=float(sy), young=float(young), poisson=float(poisson),
                           q0_mode=int(q0_mode), stiffness_q0=float(stiffness_q0),
-                          thickness=float("nan") if thickness is None else float(thickness))
+                          thickness=float("nan") if thickness is None else float(thickness),
+                          periodic_x=int(bool(periodic_x)), periodic_y=int(bool(periodic_y)),
+                          n_images=int(n_images), tile_y=int(tile_y))
         h = ctypes.c_void_p()
         check(self.lib.cm_plan_create(ctx.handle, ctypes.byref(d), ctypes.byref(h)), ctx.handle)
         self.handle = h

@@ -38,6 +38,13 @@ template <typename T, bool INV> __device__ __forceinline__ cplx<T> rot90(cplx<T>
 // inverse exp(+2 pi i/R).  Constants are the exact trigonometric values.
 template <typename T, int R, bool INV> struct Dft;
 
+// Radix 1: the identity.  Lines whose length has a prime factor above 5 start
+// and end with an identity pass, so the register-ended first and last passes
+// of the kernels stay compile-time radices (see make_line_plan).
+template <typename T, bool INV> struct Dft<T, 1, INV> {
+  __device__ __forceinline__ static void run(cplx<T>*) {}
+};
+
 template <typename T, bool INV> struct Dft<T, 2, INV> {
   __device__ __forceinline__ static void run(cplx<T>* v) {
     cplx<T> a = v[0], b = v[1];
@@ -169,15 +176,28 @@ struct LinePlan {
   int E;          // most elements a thread holds in any pass
   int npass;      // number of radix passes
   int rlast;      // radix of the last pass
-  // radix of pass s, minus one, in bits 4s..4s+3 (radices 2..16, at most 16
-  // passes): a shift instead of an indexed array, which a kernel would have
-  // to copy to private memory to index at run time
-  uint64_t rcode;
-  __host__ __device__ __forceinline__ int radix(int s) const { return (int)((rcode >> (4 * s)) & 15u) + 1; }
+  // radix of pass s in 16 bits, four passes per word (at most kMaxPasses):
+  // shifts and selects instead of an indexed array, which a kernel would
+  // have to copy to private memory to index at run time.  Radix 1 is the
+  // identity; 2, 3, 4, 5, 8 and 16 have unrolled butterflies; any other
+  // value is a generic (prime) radix.
+  uint64_t rc0, rc1, rc2;
+  static constexpr int kMaxPasses = 12;
+  __host__ __device__ __forceinline__ int radix(int s) const {
+    const uint64_t w = s < 4 ? rc0 : (s < 8 ? rc1 : rc2);
+    return (int)((w >> (16 * (s & 3))) & 0xffffu);
+  }
   __host__ void set_radix(int s, int R) {
-    rcode = (rcode & ~((uint64_t)15u << (4 * s))) | ((uint64_t)(R - 1) << (4 * s));
+    uint64_t& w = s < 4 ? rc0 : (s < 8 ? rc1 : rc2);
+    const int sh = 16 * (s & 3);
+    w = (w & ~((uint64_t)0xffffu << sh)) | ((uint64_t)R << sh);
   }
 };
+
+// Radices with unrolled butterflies (and the identity).
+__host__ __device__ __forceinline__ bool unrolled_radix(int R) {
+  return R == 1 || R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || R == 16;
+}
 
 // Twiddle table: tw[k] = exp(-2 pi i k / L), k in [0, L).
 // One Stockham pass (radix R, sub-transform length Ns before the pass).
@@ -281,20 +301,73 @@ __device__ __forceinline__ int nbutterflies(int L, int R, int tl, int tpl) {
   return full + (tl < nbf - full * tpl ? 1 : 0);
 }
 
-// One Stockham pass of radix R (R <= EMAX; radix 16 only exists in kernels
-// built for 16 or more register elements, so smaller builds stay lean).
+// Generic radix-R pass (R any value, in practice a prime above 5), computed
+// per output element instead of per butterfly: with o = q*Ns*R + r*Ns + k
+// (k < Ns, r < R) and j = q*Ns + k, the Stockham output is
+//   X[o] = sum_m x[j + m*L/R] * w^(m*(k + r*Ns)*L/(Ns*R)),  w = exp(-2 pi i/L)
+// (the pass twiddle w^(m*k*L/(Ns*R)) and the radix-R DFT factor
+// exp(-2 pi i r m/R) in one table entry; the inverse conjugates it).  Thread
+// tl < tpl owns elements tl + i*tpl, i < EMAX (the plan makes ceil(L/tpl) <=
+// EMAX); each costs R complex multiply-adds and R LDS reads, which is slow
+// for large primes but exact to the twiddle table's rounding.
+template <typename T, bool INV, int EMAX>
+__device__ void generic_pass(cplx<T>* lds, int base, int stride, int L, int Ns, int R, int tl, int tpl,
+                             const cplx<T>* __restrict__ tw) {
+  cplx<T> out[EMAX];
+  const int twstep = L / (Ns * R), LR = L / R;
+  const bool act = tl < tpl;
+#pragma unroll
+  for (int i = 0; i < EMAX; ++i) {
+    const int o = tl + i * tpl;
+    if (act && o < L) {
+      const int k = o % Ns, r = (o / Ns) % R, q = o / (Ns * R);
+      const int j = q * Ns + k;
+      const int a = (k + r * Ns) * twstep;  // < L
+      T sx = T(0), sy = T(0);
+      int idx = 0;
+      for (int m = 0; m < R; ++m) {
+        const cplx<T> x = lds[lpad<T>(base + (j + m * LR) * stride)];
+        const cplx<T> w = tw[idx];
+        const cplx<T> p = INV ? cmulc(x, w) : cmul(x, w);
+        sx += p.x;
+        sy += p.y;
+        idx += a;
+        if (idx >= L) idx -= L;
+      }
+      out[i] = mk(sx, sy);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < EMAX; ++i) {
+    const int o = tl + i * tpl;
+    if (act && o < L) lds[lpad<T>(base + o * stride)] = out[i];
+  }
+  __syncthreads();
+}
+
+// One Stockham pass of radix R (unrolled radices R <= EMAX; radix 16 only
+// exists in kernels built for 16 or more register elements, so smaller
+// builds stay lean).  Radix 1 (identity) leaves LDS as it is.
 template <typename T, bool INV, int EMAX, bool EARLY = true>
 __device__ __forceinline__ void pass_dispatch(int R, cplx<T>* lds, int base, int stride, int L, int Ns, int tl,
                                               int tpl, int nb, const cplx<T>* __restrict__ tw) {
   switch (R) {
+    case 1: break;
     case 2: stockham_pass<T, 2, INV, EMAX, EARLY>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
     case 3: stockham_pass<T, 3, INV, EMAX, EARLY>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
     case 4: stockham_pass<T, 4, INV, EMAX, EARLY>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
     case 5: stockham_pass<T, 5, INV, EMAX, EARLY>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
+    case 8: stockham_pass<T, 8, INV, EMAX, EARLY>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
     case 16:
       if constexpr (EMAX >= 16) stockham_pass<T, 16, INV, EMAX, EARLY>(lds, base, stride, L, Ns, tl, tpl, nb, tw);
       break;
-    default: stockham_pass<T, 8, INV, EMAX, EARLY>(lds, base, stride, L, Ns, tl, tpl, nb, tw); break;
+    default:
+      // generic radices only in the 16/32-element builds (the plans give
+      // such lines those builds), so the lean 8-element kernels keep their
+      // register budget
+      if constexpr (EMAX >= 16) generic_pass<T, INV, EMAX>(lds, base, stride, L, Ns, R, tl, tpl, tw);
+      break;
   }
 }
 
@@ -381,6 +454,9 @@ __device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const
   int s0 = 0, s1 = p.npass, Ns = 1;
   if (IN_REGS) {
     switch (p.radix(0)) {
+      case 1:
+        if constexpr (EMAX >= 16) first_pass_regs<T, 1, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl);
+        break;
       case 2: first_pass_regs<T, 2, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
       case 3: first_pass_regs<T, 3, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
       case 4: first_pass_regs<T, 4, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
@@ -412,6 +488,9 @@ __device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const
       return;
     }
     switch (p.rlast) {
+      case 1:
+        if constexpr (EMAX >= 16) last_pass_regs<T, 1, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw);
+        break;
       case 2: last_pass_regs<T, 2, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
       case 3: last_pass_regs<T, 3, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
       case 4: last_pass_regs<T, 4, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;

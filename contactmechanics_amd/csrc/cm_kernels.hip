@@ -1497,22 +1497,48 @@ struct ElemFinish {
 // ---------------------------------------------------------------------------
 // Green's function set-up kernels
 
-// Love's displacement under unit pressure on one pixel, on the wrapped 2N
-// grid (FFTElasticHalfSpace.py:1041-1137).
+// Love's displacement at (x, y) under unit pressure on one pixel of
+// half-widths a, b centred at the origin (Johnson 1985, p. 54).
+__device__ double love_patch(double x, double y, double a, double b, double young) {
+  const double xp = x + a, xm = x - a, yp = y + b, ym = y - b;
+  auto term = [](double p, double q, double r, double s) {
+    return p * log((q + sqrt(q * q + p * p)) / (r + sqrt(r * r + s * s)));
+  };
+  return (term(xp, yp, ym, xp) + term(yp, xp, xm, yp) + term(xm, ym, yp, xm) + term(ym, xm, xp, ym)) /
+         (M_PI * young);
+}
+
+// Free half-space: the Love kernel on the wrapped 2N grid, coordinate i*dx
+// for i <= nx and (i - 2nx)*dx above (FFTElasticHalfSpace.py:1041-1137).
 __global__ void k_love_kernel(int nx, int ny, double dx, double dy, double young, double* out) {
   const int Mx = 2 * nx, My = 2 * ny;
   const size_t M = (size_t)Mx * My;
-  const double a = dx * 0.5, b = dy * 0.5;
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < M; idx += (size_t)gridDim.x * blockDim.x) {
     const int i = (int)(idx / My), j = (int)(idx - (size_t)i * My);
-    const double x = (i <= nx ? i : i - Mx) * dx;
-    const double y = (j <= ny ? j : j - My) * dy;
-    const double xp = x + a, xm = x - a, yp = y + b, ym = y - b;
-    auto term = [](double p, double q, double r, double s) {
-      return p * log((q + sqrt(q * q + p * p)) / (r + sqrt(r * r + s * s)));
-    };
-    out[idx] = (term(xp, yp, ym, xp) + term(yp, xp, xm, yp) + term(xm, ym, yp, xm) + term(ym, xm, xp, ym)) /
-               (M_PI * young);
+    out[idx] = love_patch((i <= nx ? i : i - Mx) * dx, (j <= ny ? j : j - My) * dy, dx * 0.5, dy * 0.5, young);
+  }
+}
+
+// Semi-periodic half-space (FFTElasticHalfSpace.py:1365-1497): the Love
+// kernel at numpy-fftfreq-wrapped coordinates (i*dx for i <= (Mx-1)/2, else
+// (i - Mx)*dx), summed over the images shifted by ix*sx, |ix| <= nimx, and
+// iy*sy, |iy| <= nimy (images in the x-outer, y-inner order of the
+// reference).  The grid holds `tile` periods of My1 points along y; each holds
+// the kernel of one period divided by tile, so that a convolution over the
+// tiled grid of a field tiled the same way is the one-period convolution,
+// tiled.
+__global__ void k_semi_kernel(int Mx, int My1, int tile, double dx, double dy, double sx, double sy, int nimx,
+                              int nimy, double young, double* out) {
+  const int My = My1 * tile;
+  const size_t M = (size_t)Mx * My;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < M; idx += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / My), j = (int)(idx - (size_t)i * My) % My1;
+    const double x0 = (i <= (Mx - 1) / 2 ? i : i - Mx) * dx;
+    const double y0 = (j <= (My1 - 1) / 2 ? j : j - My1) * dy;
+    double acc = 0.0;
+    for (int ix = -nimx; ix <= nimx; ++ix)
+      for (int iy = -nimy; iy <= nimy; ++iy) acc += love_patch(x0 + ix * sx, y0 + iy * sy, dx * 0.5, dy * 0.5, young);
+    out[idx] = acc / tile;
   }
 }
 
@@ -1681,16 +1707,21 @@ void dev_free(cm_ctx* ctx, void* p, size_t bytes) {
   }
 }
 
-// Radix sequence and threads-per-line for a length L = 2^a 3^b 5^c.
-// Radices 16, 8, 4, 5, 3, 2 (largest first).  A pass of radix R has L/R
-// butterflies dealt round-robin to tpl threads; a thread holds at most
-// ceil(L/R/tpl)*R elements, which must fit the kernel's register budget emax.
+// Radix sequence and threads-per-line for a line of length L.
+// The factors 2, 3 and 5 go to unrolled radices 16, 8, 4, 5, 3, 2 (largest
+// first).  A pass of radix R has L/R butterflies dealt round-robin to tpl
+// threads; a thread holds at most ceil(L/R/tpl)*R elements, which must fit
+// the kernel's register budget emax.  Any other prime factor p gets a generic
+// pass (one thread per output element, ceil(L/tpl) <= emax; 16/32-element
+// builds only); such lines begin and end with an identity pass, so the kernels' register-ended first
+// and last passes see radix 1 (element tl + e*tpl in register slot e).
 // Prefer tpl dividing every L/R (balanced passes), then larger tpl.
 bool make_line_plan(int L, int max_tpl, int emax, LinePlan& best) {
   if (L < 2) return false;
   LinePlan lp{};
   lp.L = L;
-  int rem = L, np = 0;
+  std::vector<int> unrolled, generic;
+  int rem = L;
   while (rem > 1) {
     int pick = 0;
     for (int R : {16, 8, 4, 5, 3, 2})
@@ -1698,15 +1729,41 @@ bool make_line_plan(int L, int max_tpl, int emax, LinePlan& best) {
         pick = R;
         break;
       }
-    if (!pick || np >= 16) return false;
-    lp.set_radix(np++, pick);
+    if (!pick) break;
+    unrolled.push_back(pick);
     rem /= pick;
   }
+  for (int p = 7; rem > 1 && (long)p * p <= rem; p += 2)
+    while (rem % p == 0) {
+      generic.push_back(p);
+      rem /= p;
+    }
+  if (rem > 1) {
+    if (rem % 2 == 0 || rem % 3 == 0 || rem % 5 == 0) return false;  // a factor the budget emax left over
+    generic.push_back(rem);
+  }
+  if (!generic.empty() && emax < 16) return false;  // generic passes exist in the 16/32-element builds
+  std::vector<int> seq;
+  if (generic.empty()) {
+    seq = unrolled;
+  } else {
+    seq.push_back(1);
+    seq.insert(seq.end(), unrolled.begin(), unrolled.end());
+    seq.insert(seq.end(), generic.begin(), generic.end());
+    seq.push_back(1);
+  }
+  if ((int)seq.size() > LinePlan::kMaxPasses) return false;
+  int np = 0;
+  for (int R : seq) lp.set_radix(np++, R);
   lp.npass = np;
   lp.rlast = lp.radix(np - 1);
   int need = 1;  // smallest tpl meeting the register budget
   for (int s = 0; s < np; ++s) {
     const int R = lp.radix(s);
+    if (!unrolled_radix(R)) {
+      need = std::max(need, (L + emax - 1) / emax);
+      continue;
+    }
     const int nbmax = emax / R;
     if (nbmax < 1) return false;
     need = std::max(need, (L / R + nbmax - 1) / nbmax);
@@ -1715,7 +1772,10 @@ bool make_line_plan(int L, int max_tpl, int emax, LinePlan& best) {
   int tpl = need;
   for (int t = need; t <= max_tpl && t <= L; ++t) {
     bool balanced = true;
-    for (int s = 0; s < np; ++s) balanced = balanced && (L / lp.radix(s)) % t == 0;
+    for (int s = 0; s < np; ++s) {
+      const int R = lp.radix(s);
+      balanced = balanced && (unrolled_radix(R) ? L / R : L) % t == 0;
+    }
     if (balanced) {
       tpl = t;
       break;
@@ -1725,7 +1785,7 @@ bool make_line_plan(int L, int max_tpl, int emax, LinePlan& best) {
   int E = 0;
   for (int s = 0; s < np; ++s) {
     const int R = lp.radix(s);
-    E = std::max(E, ((L / R + tpl - 1) / tpl) * R);
+    E = std::max(E, unrolled_radix(R) ? ((L / R + tpl - 1) / tpl) * R : (L + tpl - 1) / tpl);
   }
   lp.E = E;
   best = lp;
@@ -1801,6 +1861,15 @@ struct Shape {
 };
 
 int emax_bucket(int E) { return E <= 8 ? 8 : (E <= 16 ? 16 : 32); }
+// Register build of a line plan: the smallest holding E elements, and at
+// least the 16-element one when a pass has a generic radix (the 8-element
+// builds have no generic passes).
+int plan_emax(const LinePlan& lp) {
+  int e = emax_bucket(lp.E);
+  for (int s = 0; s < lp.npass; ++s)
+    if (!unrolled_radix(lp.radix(s))) e = std::max(e, 16);
+  return e;
+}
 
 template <typename T>
 bool choose_shape(FftExec& ex, const Shape& sh) {
@@ -1817,7 +1886,7 @@ bool choose_shape(FftExec& ex, const Shape& sh) {
            make_line_plan(L, fft_block_max(32), 32, lp);
   };
   if (!plan_len(g.N2, sh.erow, ex.row) || !plan_len(Mx, sh.ecol, ex.col)) return false;
-  const int row_max = fft_block_max(emax_bucket(ex.row.E)), col_max = fft_block_max(emax_bucket(ex.col.E));
+  const int row_max = fft_block_max(plan_emax(ex.row)), col_max = fft_block_max(plan_emax(ex.col));
   // Row blocks write runs of R*Cb spectrum values: aim at >= 128-byte runs
   // with blocks that leave room for two per CU.
   int Cb = 1;
@@ -1849,8 +1918,8 @@ bool choose_shape(FftExec& ex, const Shape& sh) {
   g.R = R;
   ex.lds_row = lds_bytes<T>((size_t)R * g.N2);
   ex.lds_col = lds_bytes<T>((size_t)Mx * Cb);
-  ex.emax_row = emax_bucket(ex.row.E);
-  ex.emax_col = emax_bucket(ex.col.E);
+  ex.emax_row = plan_emax(ex.row);
+  ex.emax_col = plan_emax(ex.col);
   ex.col_rev = ex.col;
   for (int q = 0; q < ex.col.npass; ++q) ex.col_rev.set_radix(q, ex.col.radix(ex.col.npass - 1 - q));
   ex.col_rev.rlast = ex.col_rev.radix(ex.col.npass - 1);
@@ -1869,7 +1938,8 @@ int exec_setup(cm_ctx* ctx, FftExec& ex, int nx, int ny, int Mx, int My, size_t&
   if (!choose_shape<T>(ex, Shape{}))
     return fail(ctx, CM_E_UNSUPPORTED,
                 "grid " + std::to_string(Mx) + "x" + std::to_string(My) +
-                    " not supported: Mx and My/2 must factor into 2, 3, 5 and a column must fit in LDS");
+                    " not supported: a row of My/2 or a column of Mx points must fit one block (512 threads x 32 "
+                    "elements) and a column must fit in LDS");
   auto twr = host_twiddles<T>(g.N2, g.N2);
   auto twc = host_twiddles<T>(Mx, Mx);
   auto twp = host_twiddles<T>(g.N2 / 2 + 1, My);
@@ -2055,8 +2125,10 @@ void periodic_green_host(const cm_substrate_desc& d, int Mx, int My, std::vector
   }
 }
 
-// Free Green's function: Love kernel on the 2N grid, FP64 forward transform
-// on the device, real part unpacked (FFTElasticHalfSpace.py:1033-1139).
+// Free and semi-periodic Green's functions: the real-space kernel on the
+// computational grid, FP64 forward transform on the device, real part
+// unpacked (FFTElasticHalfSpace.py:1033-1139, 1353-1398; both kernels are even
+// in x and y, so their spectra are real).
 int free_green_device(cm_plan* P, std::vector<double>& G) {
   cm_ctx* ctx = P->ctx;
   FftExec ex;
@@ -2073,7 +2145,17 @@ int free_green_device(cm_plan* P, std::vector<double>& G) {
   if ((rc = dev_alloc(ctx, (void**)&Gd, gn * sizeof(double)))) return rc;
   hipStream_t st = ctx->stream;
   const double dx = P->d.sx / P->d.nx, dy = P->d.sy / P->d.ny;
-  hipLaunchKernelGGL(k_love_kernel, dim3(1024), dim3(256), 0, st, P->d.nx, P->d.ny, dx, dy, P->d.young, field);
+  if (P->d.bc == CM_FREE) {
+    hipLaunchKernelGGL(k_love_kernel, dim3(1024), dim3(256), 0, st, P->d.nx, P->d.ny, dx, dy, P->d.young, field);
+  } else {
+    // semi-periodic: images along the periodic directions; a tiled plan
+    // (two periods along y) sums the images of one period
+    const int tile = std::max(1, P->d.tile_y);
+    const int nim = std::max(0, P->d.n_images);
+    hipLaunchKernelGGL(k_semi_kernel, dim3(1024), dim3(256), 0, st, P->Mx, P->My / tile, tile, dx, dy,
+                       P->d.sx, P->d.sy / tile, P->d.periodic_x ? nim : 0, P->d.periodic_y ? nim : 0, P->d.young,
+                       field);
+  }
   InstArr<double> ia{};
   ia.arr = nullptr;
   ia.one.src = field;
@@ -2128,10 +2210,34 @@ int plan_build(cm_plan* P) {
   int rc = exec_setup<T>(ctx, P->ex, P->d.nx, P->d.ny, P->Mx, P->My, P->bytes_owned);
   if (rc) return rc;
   std::vector<double> Graw;
-  if (P->d.bc == CM_PERIODIC)
-    periodic_green_host(P->d, P->Mx, P->My, Graw);
-  else if ((rc = free_green_device(P, Graw)))
-    return rc;
+  if (P->d.bc == CM_PERIODIC) {
+    const int tile = std::max(1, P->d.tile_y);
+    if (tile == 1) {
+      periodic_green_host(P->d, P->Mx, P->My, Graw);
+    } else {
+      // the grid holds `tile` periods along y: the kernel of one period at
+      // the frequencies a tiled field has (ky a multiple of tile), zero at
+      // the others
+      cm_substrate_desc d1 = P->d;
+      d1.ny /= tile;
+      d1.sy /= tile;
+      std::vector<double> G1;
+      periodic_green_host(d1, P->Mx, P->My / tile, G1);
+      const int N2 = P->My / 2;
+      Graw.assign((size_t)(N2 + 1) * P->Mx, 0.0);
+      for (int ky = 0; ky <= N2; ky += tile)
+        std::copy(G1.begin() + (size_t)(ky / tile) * P->Mx, G1.begin() + (size_t)(ky / tile + 1) * P->Mx,
+                  Graw.begin() + (size_t)ky * P->Mx);
+    }
+  } else {
+    if ((rc = free_green_device(P, Graw))) return rc;
+    // a tiled semi-periodic grid: the transform of the tiled kernel vanishes
+    // at odd ky up to round-off; exact zeros there keep 1/G (evaluate_force)
+    // from amplifying that round-off
+    const int tile = std::max(1, P->d.tile_y);
+    for (int ky = 0; tile > 1 && ky <= P->My / 2; ++ky)
+      if (ky % tile) std::fill(Graw.begin() + (size_t)ky * P->Mx, Graw.begin() + (size_t)(ky + 1) * P->Mx, 0.0);
+  }
   if ((rc = upload_kernels<T>(P, Graw, P->G, P->K))) return rc;
   if (std::is_same<T, double>::value) {
     P->ex64 = P->ex;
@@ -2233,7 +2339,7 @@ int autotune(cm_plan* P, int reps, std::string& desc) {
   snprintf(buf, sizeof buf,
            "{\"Cb\": %d, \"R\": %d, \"erow\": %d, \"ecol\": %d, \"row_tpl\": %d, \"col_tpl\": %d, "
            "\"evaluate_us\": %.3f, \"candidates\": %d}",
-           best.g.Cb, best.g.R, emax_bucket(best.row.E), emax_bucket(best.col.E), best.row.tpl, best.col.tpl,
+           best.g.Cb, best.g.R, plan_emax(best.row), plan_emax(best.col), best.row.tpl, best.col.tpl,
            best_us, tried);
   desc = buf;
   return 0;
@@ -2514,6 +2620,9 @@ int session_init(cm_session* S, const cm_ccg_params* prm, const double* const* h
   const WsLayout& L = S->L;
   char* base = (char*)S->ws;
   const size_t M = (size_t)P->Mx * P->My;
+  if (P->d.tile_y > 1 && P->d.bc == CM_SEMI && !P->d.periodic_y)
+    return fail(ctx, CM_E_UNSUPPORTED,
+                "a plan repeating a non-periodic y has no solve (the contact set is one top-left block)");
   Inst<T>* hin = (Inst<T>*)S->hin;
   Inst<T>* hin0 = (Inst<T>*)S->hin0;
   Scal* hsc = S->host;
@@ -2912,19 +3021,34 @@ int cm_plan_create(cm_ctx* ctx, const cm_substrate_desc* d, cm_plan** out) {
   if (!ctx || !d || !out) return fail(ctx, CM_E_ARG, "null argument");
   if (d->nx < 1 || d->ny < 1) return fail(ctx, CM_E_SHAPE, "grid must be at least 1x1");
   if (!(d->sx > 0) || !(d->sy > 0)) return fail(ctx, CM_E_ARG, "physical sizes must be positive");
-  if (d->bc != CM_PERIODIC && d->bc != CM_FREE) return fail(ctx, CM_E_ARG, "bc must be CM_PERIODIC or CM_FREE");
+  if (d->bc != CM_PERIODIC && d->bc != CM_FREE && d->bc != CM_SEMI)
+    return fail(ctx, CM_E_ARG, "bc must be CM_PERIODIC, CM_FREE or CM_SEMI");
+  const int tile = std::max(1, d->tile_y);
+  if (tile > 2 || (tile == 2 && (d->ny % 2 != 0 || d->bc == CM_FREE)))
+    return fail(ctx, CM_E_ARG, "tile_y = 2 needs a periodic or semi-periodic plan and an even ny");
   if (d->precision != CM_FP64 && d->precision != CM_FP32) return fail(ctx, CM_E_ARG, "bad precision");
   (void)hipSetDevice(ctx->device);
   cm_plan* P = new cm_plan();
   P->ctx = ctx;
   P->d = *d;
   P->prec = d->precision;
-  P->Mx = d->bc == CM_FREE ? 2 * d->nx : d->nx;
-  P->My = d->bc == CM_FREE ? 2 * d->ny : d->ny;
+  if (d->bc == CM_SEMI) {
+    // Hockney padding to 2n - 1 along the non-periodic directions
+    // (FFTElasticHalfSpace.py:1344-1352); with tile_y, ny counts the points
+    // of all periods and My is tile_y computational periods
+    P->Mx = d->periodic_x ? d->nx : 2 * d->nx - 1;
+    const int ny1 = d->ny / tile;
+    P->My = tile * (d->periodic_y ? ny1 : 2 * ny1 - 1);
+  } else {
+    P->Mx = d->bc == CM_FREE ? 2 * d->nx : d->nx;
+    P->My = d->bc == CM_FREE ? 2 * d->ny : d->ny;
+  }
   P->area_per_pt = d->sx * d->sy / ((double)d->nx * d->ny);
   if (P->My % 2 != 0) {
     delete P;
-    return fail(ctx, CM_E_UNSUPPORTED, "the contiguous grid dimension My must be even");
+    return fail(ctx, CM_E_UNSUPPORTED,
+                "the contiguous grid dimension My must be even (the row transforms pack pairs of points); "
+                "an odd My is computed on the transposed grid or, along a periodic y, with tile_y = 2");
   }
   int rc = P->prec == CM_FP64 ? plan_build<double>(P) : plan_build<float>(P);
   if (rc) {
@@ -2989,8 +3113,8 @@ int cm_plan_shape(cm_plan* plan, int* Cb, int* R, int* erow, int* ecol) {
   if (!plan) return fail(nullptr, CM_E_ARG, "null plan");
   if (Cb) *Cb = plan->ex.g.Cb;
   if (R) *R = plan->ex.g.R;
-  if (erow) *erow = emax_bucket(plan->ex.row.E);
-  if (ecol) *ecol = emax_bucket(plan->ex.col.E);
+  if (erow) *erow = plan_emax(plan->ex.row);
+  if (ecol) *ecol = plan_emax(plan->ex.col);
   return 0;
 }
 

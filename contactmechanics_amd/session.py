@@ -14,6 +14,10 @@ import numpy as np
 
 from . import _native
 
+# solve statistics that are sums over the grid (scaled by the number of
+# periods a plan holds, see gridmap.py)
+_EXTENSIVE = ("fun", "area", "cg_area", "plastic_area", "total_force", "nc_count")
+
 
 def make_params(hardness=None, external_force=None, offset=None, pentol=1e-8, forcetol=1e-5,
                 thermotol=1e-6, mixfac=0.1, mixdecfac=0.95, minmixsteps=10, maxiter=100000,
@@ -42,18 +46,32 @@ class CcgSession:
         self.B = len(heights)
         dev = substrate._device_str()
         dt = substrate.torch_dtype
-        shape = tuple(substrate.nb_subdomain_grid_pts)
-        self.h = [torch.as_tensor(np.ascontiguousarray(h, dtype=np.float64)).to(dev) if not
-                  isinstance(h, torch.Tensor) else h.to(device=dev, dtype=torch.float64).contiguous()
+        # fields live on the plan's grid (the caller's unless My is odd, see
+        # gridmap.py); the load and the grid sums scale with its periods
+        self.gm = gm = substrate.gridmap
+        if not gm.solvable:
+            raise substrate.Error(f"no solve on the computational grid {substrate.nb_domain_grid_pts}: both sides "
+                                  "odd and no periodic direction")
+        shape = gm.plan_shape
+        self.h = [gm.fwd(torch.as_tensor(np.ascontiguousarray(h, dtype=np.float64)).to(dev) if not
+                         isinstance(h, torch.Tensor) else h.to(device=dev, dtype=torch.float64).contiguous())
                   for h in heights]
         if initial_displacements is None:
             self.u = [torch.zeros(shape, dtype=dt, device=dev) for _ in range(self.B)]
         else:
-            self.u = [torch.as_tensor(u).to(device=dev, dtype=dt).contiguous().clone()
+            self.u = [gm.fwd(torch.as_tensor(u).to(device=dev, dtype=dt).contiguous().clone())
                       for u in initial_displacements]
         self.f = [torch.zeros(shape, dtype=dt, device=dev) for _ in range(self.B)]
         self.cm = [torch.zeros(shape, dtype=torch.uint8, device=dev) for _ in range(self.B)]
         params = list(params)
+        if gm.tile > 1:
+            scaled = []
+            for p in params:
+                q = _native.CcgParams()
+                ctypes.pointer(q)[0] = p
+                q.external_force = p.external_force * gm.tile
+                scaled.append(q)
+            params = scaled
         self._params = (_native.CcgParams * self.B)(*params)
         vp = lambda xs: (ctypes.c_void_p * self.B)(*[ctypes.c_void_p(x.data_ptr()) for x in xs])  # noqa: E731
         self._hp, self._up, self._fp, self._cp = vp(self.h), vp(self.u), vp(self.f), vp(self.cm)
@@ -78,14 +96,25 @@ class CcgSession:
     def step(self, n):
         _native.check(self.lib.cm_ccg_step(self.handle, int(n)), self.ctx.handle)
 
+    def _stats(self):
+        out = [s.as_dict() for s in self.stats]
+        for d in out:
+            for k in _EXTENSIVE:
+                d[k] = d[k] / self.gm.tile
+        return out
+
     def poll(self):
         done = ctypes.c_int()
         _native.check(self.lib.cm_ccg_poll(self.handle, self.stats, ctypes.byref(done)), self.ctx.handle)
-        return bool(done.value), [s.as_dict() for s in self.stats]
+        return bool(done.value), self._stats()
 
     def finish(self):
         _native.check(self.lib.cm_ccg_finish(self.handle, self.stats), self.ctx.handle)
-        return [s.as_dict() for s in self.stats]
+        return self._stats()
+
+    def displacements(self, b=0):
+        """Displacements on the computational grid (``result.x``)."""
+        return self.gm.inv(self.u[b])
 
     def profile(self, n):
         out = (ctypes.c_float * 9)()
@@ -104,7 +133,7 @@ class CcgSession:
     def pressures(self, b=0):
         """Pixel forces on the topography, compressive positive (``result.jac``)."""
         nx, ny = self.substrate.nb_grid_pts
-        return -self.f[b][:nx, :ny]
+        return -self.gm.inv(self.f[b])[:nx, :ny]
 
     def close(self):
         if getattr(self, "handle", None):

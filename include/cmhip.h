@@ -41,11 +41,12 @@ extern "C" {
 #define CM_E_NAN (-3)         /* reference: RuntimeError('nan encountered.') */
 #define CM_E_OOM (-4)         /* device memory budget exceeded */
 #define CM_E_HIP (-5)         /* HIP runtime error */
-#define CM_E_UNSUPPORTED (-6) /* grid size not supported by the FFT (needs 2^a 3^b 5^c) */
+#define CM_E_UNSUPPORTED (-6) /* grid not supported (odd My, or a line too long for one block) */
 #define CM_E_BISECT (-7)      /* bearing-area bisection failed (reference: scipy bisect raising) */
 
 #define CM_PERIODIC 0
 #define CM_FREE 1
+#define CM_SEMI 2
 #define CM_FP64 0
 #define CM_FP32 1
 
@@ -54,11 +55,16 @@ typedef struct cm_plan cm_plan;
 typedef struct cm_session cm_session;
 
 /* Substrate description.  Replaces the constructor arguments of
- * PeriodicFFTElasticHalfSpace (FFTElasticHalfSpace.py:169-306) and
- * FreeFFTElasticHalfSpace (FFTElasticHalfSpace.py:912-964). */
+ * PeriodicFFTElasticHalfSpace (FFTElasticHalfSpace.py:169-306),
+ * FreeFFTElasticHalfSpace (FFTElasticHalfSpace.py:912-964) and
+ * SemiPeriodicFFTElasticHalfSpace (FFTElasticHalfSpace.py:1286-1352).
+ * Computational grid: periodic nx x ny; free 2nx x 2ny; semi-periodic n along
+ * a periodic direction, 2n - 1 along a free one.  My must be even; the host
+ * layer computes other grids on the transposed grid or (periodic y) on two
+ * periods with tile_y = 2. */
 typedef struct cm_substrate_desc {
   int nx, ny;              /* topography grid */
-  int bc;                  /* CM_PERIODIC or CM_FREE */
+  int bc;                  /* CM_PERIODIC, CM_FREE or CM_SEMI */
   int precision;           /* CM_FP64 or CM_FP32 */
   double sx, sy;           /* physical sizes */
   double young;            /* Young's modulus (contact modulus if poisson == 0) */
@@ -66,6 +72,10 @@ typedef struct cm_substrate_desc {
   int q0_mode;             /* periodic: 0 = stiffness_q0 None, 1 = stiffness_q0 == 0, 2 = given value */
   double stiffness_q0;     /* used when q0_mode == 2 */
   double thickness;        /* periodic finite thickness; <= 0 or NaN = infinite half-space */
+  int periodic_x, periodic_y; /* CM_SEMI: periodicity per direction (reference default 0, 1) */
+  int n_images;            /* CM_SEMI: periodic images per side in the kernel sum (reference default 10) */
+  int tile_y;              /* 0 or 1; 2: ny (and sy) hold two periods along a periodic y, and the
+                              Green's function is that of one period (zero at odd ky) */
 } cm_substrate_desc;
 
 /* Keyword arguments of constrained_conjugate_gradients

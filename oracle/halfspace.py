@@ -370,3 +370,57 @@ class FreeHalfSpace(PeriodicHalfSpace):
     def check(self, force=None):
         if self._check_boundaries:
             self.check_boundaries(force)
+
+
+class SemiPeriodicHalfSpace(FreeHalfSpace):
+    """Oracle ``SemiPeriodicFFTElasticHalfSpace`` (``FFTElasticHalfSpace.py:1286-1497``).
+
+    * computational grid: ``n`` points along a periodic direction, ``2n - 1``
+      along a free one (:1344-1352);
+    * kernel: the free Love kernel at numpy-fftfreq-wrapped coordinates
+      (``fftfreq(M) * M / n * size``, :1422-1433), summed over the periodic
+      images shifted by ``i * size`` for ``|i| <= n_images`` along each
+      periodic direction, x-images outer (:1363-1376), then forward FFT
+      (:1379-1383, kept complex as in the reference);
+    * ``evaluate_disp`` as the free half-space (pads an unpadded force array,
+      :1499-1520); ``evaluate_force`` zero-pads a smaller displacement array
+      (:1522-1542).
+    """
+
+    def __init__(self, nb_grid_pts, young, physical_sizes=2 * np.pi, periodicity=(False, True),
+                 n_images=10, check_boundaries=False, fft_dtype=np.float64):
+        self.periodicity = tuple(bool(p) for p in periodicity)
+        self.n_images = int(n_images)
+        PeriodicHalfSpace.__init__(self, nb_grid_pts, young, physical_sizes, fft_dtype=fft_dtype,
+                                   _green=False)
+        self._check_boundaries = check_boundaries
+        self.greens_function = self._semi_green()
+        self.surface_stiffness = self._stiffness_from_green()
+
+    @property
+    def nb_domain_grid_pts(self):
+        return tuple(n if p else 2 * n - 1 for n, p in zip(self.nb_grid_pts, self.periodicity))
+
+    def _semi_green(self):
+        mx, my = self.nb_domain_grid_pts
+        nx, ny = self.nb_grid_pts
+        sx, sy = self.physical_sizes
+        dx, dy = self._steps
+        x0 = (np.fft.fftfreq(mx) * mx / nx * sx)[:, None]
+        y0 = (np.fft.fftfreq(my) * my / ny * sy)[None, :]
+        kernel = np.zeros((mx, my))
+        images_x = range(-self.n_images, self.n_images + 1) if self.periodicity[0] else [0]
+        images_y = range(-self.n_images, self.n_images + 1) if self.periodicity[1] else [0]
+        for ix in images_x:
+            for iy in images_y:
+                kernel += love_rectangle_kernel(x0 + float(ix) * sx, y0 + float(iy) * sy, dx / 2, dy / 2,
+                                                self.young)
+        self.G_real = kernel
+        return np.fft.rfft2(kernel, axes=(1, 0))
+
+    def evaluate_force(self, disp):
+        if disp.shape != self.nb_subdomain_grid_pts:
+            padded = np.zeros(self.nb_subdomain_grid_pts)
+            padded[:disp.shape[0], :disp.shape[1]] = disp
+            return super().evaluate_force(padded)
+        return super().evaluate_force(disp)

@@ -16,7 +16,7 @@ sys.path.insert(0, ROOT)
 
 from contactmechanics_amd import _native as N  # noqa: E402  (ctypes structs only)
 from oracle.ccg import constrained_conjugate_gradients as ccg  # noqa: E402
-from oracle.halfspace import FreeHalfSpace, PeriodicHalfSpace  # noqa: E402
+from oracle.halfspace import FreeHalfSpace, PeriodicHalfSpace, SemiPeriodicHalfSpace  # noqa: E402
 
 lib = ctypes.CDLL(os.path.join(HERE, "_build", "libcmemu.so"))
 lib.cm_last_error.restype = ctypes.c_char_p
@@ -24,9 +24,10 @@ ctx = ctypes.c_void_p()
 assert lib.cm_ctx_create(0, ctypes.c_size_t(0), ctypes.byref(ctx)) == 0
 
 
-def plan(nx, ny, bc, prec, sx=1.0, sy=1.0, young=1.0):
+def plan(nx, ny, bc, prec, sx=1.0, sy=1.0, young=1.0, periodicity=(0, 1), n_images=10, tile_y=1):
     d = N.SubstrateDesc(nx=nx, ny=ny, bc=bc, precision=prec, sx=sx, sy=sy, young=young, poisson=0.0,
-                        q0_mode=0, stiffness_q0=0.0, thickness=float("nan"))
+                        q0_mode=0, stiffness_q0=0.0, thickness=float("nan"), periodic_x=periodicity[0],
+                        periodic_y=periodicity[1], n_images=n_images, tile_y=tile_y)
     p = ctypes.c_void_p()
     rc = lib.cm_plan_create(ctx, ctypes.byref(d), ctypes.byref(p))
     assert rc == 0, lib.cm_last_error(ctx)
@@ -41,8 +42,13 @@ def heights(nx, ny, seed):
 
 
 def solve(case):
+    """A solve through cm_ccg_solve against the oracle.  bc 2: semi-periodic
+    (case["per"]); "tile": the plan holds two periods along a periodic y
+    (heights repeated, load doubled) and its first period is compared."""
     nx, ny, bc, prec = case["nx"], case["ny"], case["bc"], case["prec"]
-    p, mx, my = plan(nx, ny, bc, prec)
+    per = tuple(case.get("per", (0, 1)))
+    tile = int(case.get("tile", 1))
+    p, mx, my = plan(nx, ny * tile, bc, prec, sy=float(tile), periodicity=per, n_images=3, tile_y=tile)
     h = heights(nx, ny, case["seed"])
     kw = dict(external_force=case["force"], pentol=case["pentol"], forcetol=case["forcetol"], maxiter=600)
     if case.get("hardness"):
@@ -51,17 +57,20 @@ def solve(case):
     u = np.zeros((mx, my), dt)
     f = np.zeros((mx, my), dt)
     cm = np.zeros((mx, my), np.uint8)
-    prm = N.CcgParams(load_control=1, external_force=kw["external_force"], offset=float("nan"),
+    prm = N.CcgParams(load_control=1, external_force=kw["external_force"] * tile, offset=float("nan"),
                       has_hardness=int("hardness" in kw), hardness=kw.get("hardness", 0.0),
                       pentol=kw["pentol"], forcetol=kw["forcetol"], thermotol=1e-6, mixfac=0.1, mixdecfac=0.95,
                       minmixsteps=10, maxiter=kw["maxiter"], init_mode=0, iters_per_sync=0, use_graph=0)
     st = N.CcgStats()
-    hc = np.ascontiguousarray(h)
+    hc = np.ascontiguousarray(np.tile(h, (1, tile)))
     vp = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
     rc = lib.cm_ccg_solve(p, ctypes.byref(prm), vp(hc), None, vp(u), vp(f), vp(cm), ctypes.byref(st),
                           N.PROGRESS_FN(), None, None)
-    cls = FreeHalfSpace if bc else PeriodicHalfSpace
-    r = ccg(cls((nx, ny), 1.0, (1.0, 1.0)), h, **kw)
+    if bc == 2:
+        sub = SemiPeriodicHalfSpace((nx, ny), 1.0, (1.0, 1.0), periodicity=per, n_images=3)
+    else:
+        sub = (FreeHalfSpace if bc else PeriodicHalfSpace)((nx, ny), 1.0, (1.0, 1.0))
+    r = ccg(sub, h, **kw)
     jac = -f[:nx, :ny].astype(np.float64)
     return dict(rc=rc, nit=st.nit, nit_oracle=int(r.nit), success_oracle=bool(r.success),
                 rel=float(np.max(np.abs(jac - r.jac)) / np.max(np.abs(r.jac))))
@@ -128,11 +137,23 @@ def kspace(case):
 
 
 def operators(case):
+    """Operators of every launch shape against the oracle.  bc 2: semi-periodic
+    with case["per"] = (periodic_x, periodic_y); "tile": the plan holds two
+    periods along y (periodic y) and the field is the oracle's, tiled."""
     nx, ny, bc = case["nx"], case["ny"], case["bc"]
-    p, mx, my = plan(nx, ny, bc, 0, 1.0, 2.0, 1.7)
-    c = (FreeHalfSpace if bc else PeriodicHalfSpace)((nx, ny), 1.7, (1.0, 2.0))
-    f = np.random.default_rng(nx + ny).standard_normal((mx, my))
-    ud, fd = c.evaluate_disp(f), c.evaluate_force(f)
+    per = tuple(case.get("per", (0, 1)))
+    tile = int(case.get("tile", 1))
+    if bc == 2:
+        c = SemiPeriodicHalfSpace((nx, ny), 1.7, (1.0, 2.0), periodicity=per, n_images=case.get("images", 3))
+    else:
+        c = (FreeHalfSpace if bc else PeriodicHalfSpace)((nx, ny), 1.7, (1.0, 2.0))
+    p, mx, my = plan(nx, ny * tile, bc, 0, 1.0, 2.0 * tile, 1.7, periodicity=per, n_images=case.get("images", 3),
+                     tile_y=tile)
+    f1 = np.random.default_rng(nx + ny).standard_normal(c.nb_domain_grid_pts)
+    ud, fd = c.evaluate_disp(f1), c.evaluate_force(f1)
+    f = np.ascontiguousarray(np.tile(f1, (1, tile)))
+    ud, fd = np.tile(ud, (1, tile)), np.tile(fd, (1, tile))
+    assert (mx, my) == f.shape, ((mx, my), f.shape)
     worst, n = 0.0, 0
     for sh in [None] + list(itertools.product((1, 2, 4, 8), (1, 2, 4, 8), (8, 16, 32), (8, 16, 32))):
         if sh is not None and lib.cm_plan_set_shape(p, *sh) != 0:

@@ -90,3 +90,37 @@ def test_emulated_kspace_helpers(emu_lib, nx, ny, bc):
     tol = 1e-13 if bc == 0 else 1e-12
     for k, v in r.items():
         assert v < tol, (k, v)
+
+
+@pytest.mark.parametrize("case", [
+    dict(nx=21, ny=22, bc=0),                     # Mx = 3*7, My/2 = 11: generic radices on both axes
+    dict(nx=8, ny=14, bc=0),                      # My/2 = 7 (16-element build for a short generic line)
+    dict(nx=7, ny=11, bc=1),                      # free 14 x 22
+    dict(nx=6, ny=8, bc=2, per=[0, 1]),           # semi-periodic 11 x 8
+    dict(nx=5, ny=4, bc=2, per=[1, 1]),           # semi-periodic, images along both axes
+    dict(nx=8, ny=7, bc=0, tile=2),               # odd periodic y: two periods, one-period kernel
+    dict(nx=6, ny=7, bc=2, per=[0, 1], tile=2),   # semi-periodic 11 x 7 on two periods
+    dict(nx=6, ny=7, bc=2, per=[0, 0], tile=2),   # semi-periodic 11 x 13: operators only
+])
+def test_emulated_operators_any_grid(emu_lib, case):
+    # grids with prime factors above 5 (generic Stockham passes between
+    # identity end passes), the semi-periodic kernel, and plans holding two
+    # periods of an odd y, against the oracle at every launch shape
+    r = run_case(dict(kind="ops", **case))
+    assert r["shapes"] >= 1
+    assert r["worst"] < 1e-11
+
+
+@pytest.mark.parametrize("case", [
+    dict(nx=21, ny=22, bc=0, prec=0, seed=3),
+    dict(nx=8, ny=7, bc=0, tile=2, prec=0, seed=4),
+    dict(nx=12, ny=10, bc=2, per=[0, 1], prec=0, seed=5),
+    dict(nx=10, ny=9, bc=2, per=[0, 1], tile=2, prec=1, seed=6),
+])
+def test_emulated_solve_any_grid(emu_lib, case):
+    r = run_case(dict(kind="solve", force=3e-3, pentol=1e-10, forcetol=1e-14, **case))
+    assert r["rc"] == 0 and r["success_oracle"]
+    if case["prec"] == 0:
+        assert r["nit"] == r["nit_oracle"] and r["rel"] < 1e-11
+    else:
+        assert abs(r["nit"] - r["nit_oracle"]) <= 1 and r["rel"] < 1e-7

@@ -77,9 +77,10 @@ def test_torch_device_path_and_errors(gpu):
     assert rel(ud.cpu().numpy(), c.evaluate_disp(f)) < 1e-13
     with pytest.raises(g.Error):
         g.evaluate_disp(np.zeros((63, 64)))
-    # 7 and 14 are not 2^a 3^b 5^c: the plan refuses loudly (no CPU fallback)
+    # a row of 16411 complex points (prime) needs more than 512 threads x 32
+    # elements: the plan refuses loudly (no CPU fallback)
     with pytest.raises(PeriodicFFTElasticHalfSpace.Error):
-        PeriodicFFTElasticHalfSpace((7, 14), 1.0, (1.0, 1.0)).evaluate_disp(np.zeros((7, 14)))
+        PeriodicFFTElasticHalfSpace((4, 2 * 16411), 1.0, (1.0, 1.0)).evaluate_disp(np.zeros((4, 2 * 16411)))
 
 
 @pytest.mark.parametrize("nx,ny", [(4000, 32), (32, 4000), (2430, 48)])

@@ -36,8 +36,29 @@ def test_library_exports_every_declared_symbol():
     assert lib.cm_version().decode().startswith("cmhip")
 
 
-def test_ctypes_struct_layouts_match_header():
+def test_ctypes_struct_layouts_match_header(tmp_path):
+    """Sizes and field offsets of the ctypes structs equal the C compiler's
+    for include/cmhip.h."""
     import ctypes
-    # int,int,int,int,double,double,double,double,int,(pad)double,double
-    assert ctypes.sizeof(_native.SubstrateDesc) == 4 * 4 + 4 * 8 + 8 + 2 * 8
-    assert ctypes.sizeof(_native.CcgStats) == 4 * 4 + 14 * 8
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    structs = {"cm_substrate_desc": _native.SubstrateDesc, "cm_ccg_params": _native.CcgParams,
+               "cm_ccg_stats": _native.CcgStats}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "cmhip.h"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(line.split()[:2]): int(line.split()[2]) for line in out if line.strip()}
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
