@@ -8,20 +8,37 @@ import sys
 
 
 class Logger:
-    def __init__(self, fobj=None):
+    """``fobj``: a file object, a file name (opened for writing) or None
+    (drop everything).  The column header is printed before the first row
+    and again every ``sepevery`` rows."""
+
+    def __init__(self, fobj=None, sepevery=10):
+        if isinstance(fobj, str):
+            fobj = open(fobj, "w")
         self.fobj = fobj
-        self._header_done = False
+        self.sepevery = sepevery
+        self._rows = 0
 
     def pr(self, msg):
         if self.fobj is not None:
             print(f"# {msg}", file=self.fobj)
+            self.fobj.flush()
+
+    def warn(self, msg):
+        self.pr("Warning: " + msg)
+        if self.fobj is not sys.stdout:
+            print(f"# Warning: {msg}", file=sys.stdout)
+
+    def flush(self):
+        if self.fobj is not None:
+            self.fobj.flush()
 
     def st(self, headers, values, force_print=False):
         if self.fobj is None:
             return
-        if not self._header_done:
+        if self._rows % max(1, self.sepevery) == 0:
             print("#" + " ".join(f"{i + 1:>5}:{h:>16}" for i, h in enumerate(headers)), file=self.fobj)
-            self._header_done = True
+        self._rows += 1
         cols = []
         for v in values:
             if isinstance(v, str):
@@ -31,6 +48,7 @@ class Logger:
             else:
                 cols.append(f"{float(v):>22.12e}")
         print(" ".join(cols), file=self.fobj)
+        self.fobj.flush()
 
     @property
     def active(self):
