@@ -62,7 +62,7 @@ def test_plastic_cone_like_reference(gpu, tmp_path):
 
 def test_displacement_and_automatic_programmes(gpu, tmp_path, monkeypatch):
     fn, h = sphere_file(tmp_path, n=32)
-    rc = HardWall.main([fn, "--physical_sizes", "1,1", "--displacement", "-0.02,0.0,3", "--log-fn",
+    rc = HardWall.main([fn, "--physical_sizes", "1,1", "--displacement=-0.02,0.0,3", "--log-fn",
                         str(tmp_path / "d.out"), "--netcdf-fn", str(tmp_path / "d.nc")], logger=quiet)
     assert rc == 0 and len(NetCDFContainer(str(tmp_path / "d.nc"))) == 3
     monkeypatch.setattr(HardWall, "NSTEPS", 6)
