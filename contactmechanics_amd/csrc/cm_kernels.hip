@@ -265,9 +265,17 @@ __device__ __forceinline__ void st2(T* p, T a, T b) {
   v[1] = b;
   *(__attribute__((address_space(1))) Gv2<T>*)p = v;
 }
-__device__ __forceinline__ P2<uint8_t> ldm(const uint8_t* p) {
-  const uint16_t v = *(const __attribute__((address_space(1))) uint16_t*)p;
-  return P2<uint8_t>{(uint8_t)(v & 0xff), (uint8_t)(v >> 8)};
+// Contact-mask bytes of a pair, kept as the loaded 16-bit word until used:
+// splitting it right after the load makes the compiler wait for that load,
+// and with in-order memory counters that drains every load still in flight,
+// so a batch of point pairs would go to memory one pair at a time.
+struct M2 {
+  uint16_t v;
+  __device__ __forceinline__ uint8_t a() const { return (uint8_t)(v & 0xff); }
+  __device__ __forceinline__ uint8_t b() const { return (uint8_t)(v >> 8); }
+};
+__device__ __forceinline__ M2 ldm(const uint8_t* p) {
+  return M2{*(const __attribute__((address_space(1))) uint16_t*)p};
 }
 __device__ __forceinline__ void stm(uint8_t* p, uint8_t a, uint8_t b) {
   *(__attribute__((address_space(1))) uint16_t*)p = (uint16_t)(a | (b << 8));
@@ -434,21 +442,38 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_r2
   {
     const int R0 = lp.radix(0);
     const int nb0 = nbutterflies(N2, R0, tl, lp.tpl);
+    if (nb0 * R0 >= EMAX) {
+      // every register slot holds a point pair: no per-slot branch between
+      // the loads of a batch, so they all issue before the first is used
 #pragma unroll
-    for (int e0 = 0; e0 < EMAX; e0 += kPairs) {
-      typename P::Rec rec[kPairs];
+      for (int e0 = 0; e0 < EMAX; e0 += kPairs) {
+        typename P::Rec rec[kPairs];
 #pragma unroll
-      for (int k = 0; k < kPairs; ++k) {
-        const int e = e0 + k;
-        if (e / R0 < nb0) rec[k] = p.fetch(g, row, 2 * slot_element(e, R0, N2, tl, lp.tpl));
-      }
+        for (int k = 0; k < kPairs; ++k) rec[k] = p.fetch(g, row, 2 * slot_element(e0 + k, R0, N2, tl, lp.tpl));
 #pragma unroll
-      for (int k = 0; k < kPairs; ++k) {
-        const int e = e0 + k;
-        if (e / R0 < nb0) {
+        for (int k = 0; k < kPairs; ++k) {
           T a, b;
-          p.make(g, row, 2 * slot_element(e, R0, N2, tl, lp.tpl), rec[k], a, b, acc);
-          v[e] = mk(a, b);
+          p.make(g, row, 2 * slot_element(e0 + k, R0, N2, tl, lp.tpl), rec[k], a, b, acc);
+          v[e0 + k] = mk(a, b);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int e0 = 0; e0 < EMAX; e0 += kPairs) {
+        typename P::Rec rec[kPairs];
+#pragma unroll
+        for (int k = 0; k < kPairs; ++k) {
+          const int e = e0 + k;
+          if (e / R0 < nb0) rec[k] = p.fetch(g, row, 2 * slot_element(e, R0, N2, tl, lp.tpl));
+        }
+#pragma unroll
+        for (int k = 0; k < kPairs; ++k) {
+          const int e = e0 + k;
+          if (e / R0 < nb0) {
+            T a, b;
+            p.make(g, row, 2 * slot_element(e, R0, N2, tl, lp.tpl), rec[k], a, b, acc);
+            v[e] = mk(a, b);
+          }
         }
       }
     }
@@ -645,6 +670,19 @@ __device__ __forceinline__ void c2r_finish(const P& p, const Geom& g, int row, i
   const typename P::Ctx cx = p.ctx();
   last_pass_regs<T, R, true, EMAX>(v, lds, rb, 1, N2, tl, lp.tpl, tw);
   const int nbl = nbutterflies(N2, R, tl, lp.tpl);
+  if (nbl * R >= EMAX) {
+    // every slot holds a pair: the batch's loads issue with no branch between
+#pragma unroll
+    for (int e0 = 0; e0 < EMAX; e0 += P::kBatch) {
+      typename P::Rec rec[P::kBatch];
+#pragma unroll
+      for (int k = 0; k < P::kBatch; ++k) rec[k] = p.fetch(g, row, 2 * slot_element(e0 + k, R, N2, tl, lp.tpl));
+#pragma unroll
+      for (int k = 0; k < P::kBatch; ++k)
+        p.take(g, row, 2 * slot_element(e0 + k, R, N2, tl, lp.tpl), rec[k], v[e0 + k].x, v[e0 + k].y, acc, cx);
+    }
+    return;
+  }
 #pragma unroll
   for (int e0 = 0; e0 < EMAX; e0 += P::kBatch) {
     typename P::Rec rec[P::kBatch];
@@ -770,6 +808,15 @@ __global__ void __launch_bounds__(256) k_elem(Geom g, InstArr<T> ia, int ctr_slo
   for (int row = blockIdx.x; row < g.Mx; row += gridDim.x) {
     for (int n0 = threadIdx.x; n0 < npair; n0 += kPairs * blockDim.x) {
       typename P::Rec rec[kPairs];
+      if (n0 + (kPairs - 1) * (int)blockDim.x < npair) {
+        // a full batch: no per-pair branch between the loads, so every load
+        // of the batch issues before the first value is used
+#pragma unroll
+        for (int k = 0; k < kPairs; ++k) rec[k] = p.fetch(g, row, 2 * (n0 + k * (int)blockDim.x));
+#pragma unroll
+        for (int k = 0; k < kPairs; ++k) p.emit(g, row, 2 * (n0 + k * (int)blockDim.x), rec[k], acc);
+        continue;
+      }
 #pragma unroll
       for (int k = 0; k < kPairs; ++k) {
         const int n = n0 + k * blockDim.x;
@@ -1112,7 +1159,7 @@ struct ProdDir {
   struct Rec {
     P2<double> w;
     P2<T> t;
-    P2<uint8_t> cm;
+    M2 cm;
     CPair cp;
   };
   const Inst<T> in;  // by value: the field pointers stay in registers (stores cannot alias them)
@@ -1136,8 +1183,8 @@ struct ProdDir {
   }
   __device__ void make(const Geom& g, int row, int col, const Rec& r, T& a, T& b, double* acc) const {
     const Scal* s = &sv;
-    a = one(s, r.cp.c0, r.cm.a & kNext, r.w.a, r.t.a, acc);
-    b = one(s, r.cp.c1, r.cm.b & kNext, r.w.b, r.t.b, acc);
+    a = one(s, r.cp.c0, r.cm.a() & kNext, r.w.a, r.t.a, acc);
+    b = one(s, r.cp.c1, r.cm.b() & kNext, r.w.b, r.t.b, acc);
     st2(in.t + (size_t)row * g.My + col, a, b);
   }
   __device__ void finalize(double* a) const { in.sc->Scgt = a[0]; }
@@ -1177,7 +1224,7 @@ struct ElemPhaseB {
     P2<double> f;
     P2<T> t;
     P2<double> w;
-    P2<uint8_t> cm;
+    M2 cm;
     CPair cp;
   };
   const Inst<T> in;  // by value: the field pointers stay in registers (stores cannot alias them)
@@ -1216,8 +1263,8 @@ struct ElemPhaseB {
   }
   __device__ void emit(const Geom& g, int row, int col, const Rec& r, double* acc) const {
     const Scal* s = &sv;
-    const uint8_t c0 = one(s, r.cp.c0, r.cm.a, r.f.a, (double)r.t.a, r.w.a, acc);
-    const uint8_t c1 = one(s, r.cp.c1, r.cm.b, r.f.b, (double)r.t.b, r.w.b, acc);
+    const uint8_t c0 = one(s, r.cp.c0, r.cm.a(), r.f.a, (double)r.t.a, r.w.a, acc);
+    const uint8_t c1 = one(s, r.cp.c1, r.cm.b(), r.f.b, (double)r.t.b, r.w.b, acc);
     if (s->mode == MODE_MIX) stm(in.cm + (size_t)row * g.My + col, c0, c1);
   }
   __device__ void finalize(double* a) const {
@@ -1267,7 +1314,7 @@ struct ProdForce {
     P2<double> f;
     P2<T> t;
     P2<double> w;
-    P2<uint8_t> cm;
+    M2 cm;
     CPair cp;
   };
   const Inst<T> in;  // by value: the field pointers stay in registers (stores cannot alias them)
@@ -1309,7 +1356,7 @@ struct ProdForce {
   }
   __device__ void make(const Geom& g, int row, int col, const Rec& r, TT& a, TT& b, double* acc) const {
     const Scal* s = &sv;
-    uint8_t m0 = r.cm.a, m1 = r.cm.b;
+    uint8_t m0 = r.cm.a(), m1 = r.cm.b();
     const double f0 = one(s, r.cp.c0, m0, r.f.a, (double)r.t.a, r.w.a, acc);
     const double f1 = one(s, r.cp.c1, m1, r.f.b, (double)r.t.b, r.w.b, acc);
     const size_t p = (size_t)row * g.My + col;
@@ -1352,7 +1399,7 @@ struct ConsDisp {
   __device__ static constexpr int op(int i) { return i < 2 ? RMAX : phase_a_op(i - 2); }
   struct Rec {
     P2<double> w;
-    P2<uint8_t> cm;
+    M2 cm;
     HPair h;
   };
   const Inst<T> in;  // by value: the field pointers stay in registers (stores cannot alias them)
@@ -1406,8 +1453,8 @@ struct ConsDisp {
   }
   __device__ void take(const Geom& g, int row, int col, const Rec& r, TT a, TT b, double* acc, const Ctx& cx) const {
     double w0, w1;
-    one(cx, r.h.c0, r.cm.a, r.w.a, r.h.h0, a, w0, acc);
-    one(cx, r.h.c1, r.cm.b, r.w.b, r.h.h1, b, w1, acc);
+    one(cx, r.h.c0, r.cm.a(), r.w.a, r.h.h0, a, w0, acc);
+    one(cx, r.h.c1, r.cm.b(), r.w.b, r.h.h1, b, w1, acc);
     st2(in.w + (size_t)row * g.My + col, w0, w1);
   }
   __device__ void finalize(double* a) const {
