@@ -348,8 +348,10 @@ __device__ void generic_pass(cplx<T>* lds, int base, int stride, int L, int Ns, 
 
 // One Stockham pass of radix R (unrolled radices R <= EMAX; radix 16 only
 // exists in kernels built for 16 or more register elements, so smaller
-// builds stay lean).  Radix 1 (identity) leaves LDS as it is.
-template <typename T, bool INV, int EMAX, bool EARLY = true>
+// builds stay lean).  Radix 1 (identity) leaves LDS as it is.  Generic
+// radices exist only in the builds with GEN (lines with a prime factor above
+// 5 launch those), so the other builds keep their register budget.
+template <typename T, bool INV, int EMAX, bool EARLY = true, bool GEN = false>
 __device__ __forceinline__ void pass_dispatch(int R, cplx<T>* lds, int base, int stride, int L, int Ns, int tl,
                                               int tpl, int nb, const cplx<T>* __restrict__ tw) {
   switch (R) {
@@ -363,24 +365,21 @@ __device__ __forceinline__ void pass_dispatch(int R, cplx<T>* lds, int base, int
       if constexpr (EMAX >= 16) stockham_pass<T, 16, INV, EMAX, EARLY>(lds, base, stride, L, Ns, tl, tpl, nb, tw);
       break;
     default:
-      // generic radices only in the 16/32-element builds (the plans give
-      // such lines those builds), so the lean 8-element kernels keep their
-      // register budget
-      if constexpr (EMAX >= 16) generic_pass<T, INV, EMAX>(lds, base, stride, L, Ns, R, tl, tpl, tw);
+      if constexpr (GEN) generic_pass<T, INV, EMAX>(lds, base, stride, L, Ns, R, tl, tpl, tw);
       break;
   }
 }
 
 // Full in-place LDS FFT of one line (all threads of the block must call it;
 // threads whose tl >= tpl must pass active=false and still hit the barriers).
-template <typename T, bool INV, int EMAX, bool EARLY = true>
+template <typename T, bool INV, int EMAX, bool EARLY = true, bool GEN = false>
 __device__ void fft_line(cplx<T>* lds, int base, int stride, const LinePlan& p, int tl, bool active,
                          const cplx<T>* __restrict__ tw) {
   int Ns = 1;
   for (int s = 0; s < p.npass; ++s) {
     const int R = p.radix(s);
     const int nb = active ? nbutterflies(p.L, R, tl, p.tpl) : 0;
-    pass_dispatch<T, INV, EMAX, EARLY>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw);
+    pass_dispatch<T, INV, EMAX, EARLY, GEN>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw);
     Ns *= R;
   }
 }
@@ -448,14 +447,14 @@ __device__ __forceinline__ void last_pass_regs(cplx<T>* v, const cplx<T>* lds, i
 // in LDS (natural order), and the output left in registers (OUT_REGS,
 // last-pass order) or in LDS (natural order).  With OUT_REGS the caller must
 // synchronise before writing LDS again.
-template <typename T, bool INV, int EMAX, bool IN_REGS, bool OUT_REGS, bool EARLY = true>
+template <typename T, bool INV, int EMAX, bool IN_REGS, bool OUT_REGS, bool EARLY = true, bool GEN = false>
 __device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const LinePlan& p, int tl,
                            const cplx<T>* __restrict__ tw) {
   int s0 = 0, s1 = p.npass, Ns = 1;
   if (IN_REGS) {
     switch (p.radix(0)) {
       case 1:
-        if constexpr (EMAX >= 16) first_pass_regs<T, 1, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl);
+        if constexpr (GEN) first_pass_regs<T, 1, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl);
         break;
       case 2: first_pass_regs<T, 2, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
       case 3: first_pass_regs<T, 3, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl); break;
@@ -474,7 +473,7 @@ __device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const
   for (int s = s0; s < s1; ++s) {
     const int R = p.radix(s);
     const int nb = nbutterflies(p.L, R, tl, p.tpl);
-    pass_dispatch<T, INV, EMAX, EARLY>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw);
+    pass_dispatch<T, INV, EMAX, EARLY, GEN>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nb, tw);
     Ns *= R;
   }
   if (OUT_REGS) {
@@ -489,7 +488,7 @@ __device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const
     }
     switch (p.rlast) {
       case 1:
-        if constexpr (EMAX >= 16) last_pass_regs<T, 1, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw);
+        if constexpr (GEN) last_pass_regs<T, 1, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw);
         break;
       case 2: last_pass_regs<T, 2, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
       case 3: last_pass_regs<T, 3, INV, EMAX>(v, lds, base, stride, p.L, tl, p.tpl, tw); break;
@@ -506,13 +505,13 @@ __device__ void fft_line_x(cplx<T>* v, cplx<T>* lds, int base, int stride, const
 // All passes but the last, from LDS (natural order) to LDS; the caller runs
 // the last pass itself (last_pass_regs with its radix as a template argument,
 // so the register slots it fills are compile-time indices).
-template <typename T, bool INV, int EMAX, bool EARLY = true>
+template <typename T, bool INV, int EMAX, bool EARLY = true, bool GEN = false>
 __device__ __forceinline__ void fft_line_to_last(cplx<T>* lds, int base, int stride, const LinePlan& p, int tl,
                                  const cplx<T>* __restrict__ tw) {
   int Ns = 1;
   for (int s = 0; s + 1 < p.npass; ++s) {
     const int R = p.radix(s);
-    pass_dispatch<T, INV, EMAX, EARLY>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nbutterflies(p.L, R, tl, p.tpl), tw);
+    pass_dispatch<T, INV, EMAX, EARLY, GEN>(R, lds, base, stride, p.L, Ns, tl, p.tpl, nbutterflies(p.L, R, tl, p.tpl), tw);
     Ns *= R;
   }
 }

@@ -239,8 +239,9 @@ __device__ __forceinline__ bool reduce_blocks(double* acc, double* part, unsigne
 //   take(g, row, col, rec, a, b, acc, cx)  consumer of an FFT output pair
 //                                          (cx = ctx(): scalars read once per thread);
 //   emit(g, row, col, rec, acc)            element-wise kernel body.
-// Kernels fetch a batch of kPairs pairs per thread before computing any, so
-// the loads are in flight together instead of waiting behind stores.
+// Kernels fetch a batch of pairs per thread (kPairs in the element-wise
+// kernel, a functor's kBatch in the row passes) before computing any, so the
+// loads are in flight together instead of waiting behind stores.
 constexpr int kPairs = 4;
 
 // Field pairs are accessed as aligned 2-vectors in the global address space
@@ -417,7 +418,7 @@ __host__ __device__ constexpr int fft_block_max(int) { return 512; }
 __host__ __device__ constexpr int fft_min_waves(int emax) { return emax <= 8 ? 4 : 1; }
 
 // Row pass: real rows -> packed half spectrum (forward, unnormalised).
-template <typename T, int EMAX, class P>
+template <typename T, int EMAX, class P, bool GEN = false>
 __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_r2c(Geom g, LinePlan lp, const cplx<T>* __restrict__ tw,
                                              const cplx<T>* __restrict__ twp, InstArr<typename P::F> ia,
                                              int ctr_slot) {
@@ -442,16 +443,17 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_r2
   {
     const int R0 = lp.radix(0);
     const int nb0 = nbutterflies(N2, R0, tl, lp.tpl);
+    constexpr int KB = P::kBatch;
     if (nb0 * R0 >= EMAX) {
       // every register slot holds a point pair: no per-slot branch between
       // the loads of a batch, so they all issue before the first is used
 #pragma unroll
-      for (int e0 = 0; e0 < EMAX; e0 += kPairs) {
-        typename P::Rec rec[kPairs];
+      for (int e0 = 0; e0 < EMAX; e0 += KB) {
+        typename P::Rec rec[KB];
 #pragma unroll
-        for (int k = 0; k < kPairs; ++k) rec[k] = p.fetch(g, row, 2 * slot_element(e0 + k, R0, N2, tl, lp.tpl));
+        for (int k = 0; k < KB; ++k) rec[k] = p.fetch(g, row, 2 * slot_element(e0 + k, R0, N2, tl, lp.tpl));
 #pragma unroll
-        for (int k = 0; k < kPairs; ++k) {
+        for (int k = 0; k < KB; ++k) {
           T a, b;
           p.make(g, row, 2 * slot_element(e0 + k, R0, N2, tl, lp.tpl), rec[k], a, b, acc);
           v[e0 + k] = mk(a, b);
@@ -459,15 +461,15 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_r2
       }
     } else {
 #pragma unroll
-      for (int e0 = 0; e0 < EMAX; e0 += kPairs) {
-        typename P::Rec rec[kPairs];
+      for (int e0 = 0; e0 < EMAX; e0 += KB) {
+        typename P::Rec rec[KB];
 #pragma unroll
-        for (int k = 0; k < kPairs; ++k) {
+        for (int k = 0; k < KB; ++k) {
           const int e = e0 + k;
           if (e / R0 < nb0) rec[k] = p.fetch(g, row, 2 * slot_element(e, R0, N2, tl, lp.tpl));
         }
 #pragma unroll
-        for (int k = 0; k < kPairs; ++k) {
+        for (int k = 0; k < KB; ++k) {
           const int e = e0 + k;
           if (e / R0 < nb0) {
             T a, b;
@@ -478,7 +480,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_r2
       }
     }
   }
-  fft_line_x<T, false, EMAX, true, false>(v, lds, rb, 1, lp, tl, tw);
+  fft_line_x<T, false, EMAX, true, false, true, GEN>(v, lds, rb, 1, lp, tl, tw);
   // split the half-length transform into the real-input spectrum X[k],
   // k = 0..N2, packing X[0] (real) and X[N2] (real) into slot 0.
   for (int k = tl; k <= N2 / 2; k += lp.tpl) {
@@ -532,7 +534,7 @@ struct TauRed {
 // the step length tau from it (no output).
 // skip: 0 never, 1 when the solve is done, 2 when done or in a mixing step.
 // T: transform precision; TF: field precision of the instances.
-template <typename T, int EMAX, int MODE, typename TF>
+template <typename T, int EMAX, int MODE, typename TF, bool GEN = false>
 __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_col(Geom g, LinePlan lp, LinePlan rv, const cplx<T>* __restrict__ tw,
                                               InstArr<TF> ia, int useK, int skip) {
   extern __shared__ __align__(16) unsigned char smem_col[];
@@ -564,7 +566,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_co
 #pragma unroll
       for (int e = 0; e < EMAX; ++e)
         if (e / R0 < nb0) v[e] = Sg[base + (size_t)slot_element(e, R0, Mx, tl, lp.tpl) * Cb + c];
-      fft_line_x<T, false, EMAX, true, true, kColEarlyTw<T>>(v, lds, c, Cb, lp, tl, tw);
+      fft_line_x<T, false, EMAX, true, true, kColEarlyTw<T>, GEN>(v, lds, c, Cb, lp, tl, tw);
       const auto Gk = (const __attribute__((address_space(1))) T*)(G + (size_t)(cb * Cb + c) * Mx);
 #pragma unroll
       for (int e = 0; e < EMAX; ++e)
@@ -575,7 +577,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_co
     } else {
       slab_load(lds, S, base, n);
       __syncthreads();
-      fft_line<T, false, EMAX, kColEarlyTw<T>>(lds, c, Cb, lp, tl, true, tw);
+      fft_line<T, false, EMAX, kColEarlyTw<T>, GEN>(lds, c, Cb, lp, tl, true, tw);
       if (c != 0) {
         const T* Gk = G + (size_t)c * Mx;
         for (int x = tl; x < Mx; x += lp.tpl) {
@@ -612,7 +614,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_co
 #pragma unroll
     for (int e = 0; e < EMAX; ++e)
       if (e / R0 < nb0) v[e] = Sg[base + (size_t)slot_element(e, R0, Mx, tl, lp.tpl) * Cb + c];
-    fft_line_x<T, false, EMAX, true, true, kColEarlyTw<T>>(v, lds, c, Cb, lp, tl, tw);
+    fft_line_x<T, false, EMAX, true, true, kColEarlyTw<T>, GEN>(v, lds, c, Cb, lp, tl, tw);
     __syncthreads();
     const auto Gk = (const __attribute__((address_space(1))) T*)(G + (size_t)(cb * Cb + c) * Mx);
 #pragma unroll
@@ -620,7 +622,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_co
       if (e / Rl < nbl) v[e] = v[e] * Gk[slot_element(e, Rl, Mx, tl, lp.tpl)];
     // rv: the plan with its radices reversed, so the inverse transform's first
     // pass takes the forward transform's last-pass register slots as they are
-    fft_line_x<T, true, EMAX, true, true, kColEarlyTw<T>>(v, lds, c, Cb, rv, tl, tw);
+    fft_line_x<T, true, EMAX, true, true, kColEarlyTw<T>, GEN>(v, lds, c, Cb, rv, tl, tw);
     const GViewW<T> Sw(S);
 #pragma unroll
     for (int e = 0; e < EMAX; ++e)
@@ -630,7 +632,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_co
   slab_load(lds, S, base, n);
   __syncthreads();
   const int c = threadIdx.x % Cb, tl = threadIdx.x / Cb;
-  if (MODE != 2) fft_line<T, false, EMAX, kColEarlyTw<T>>(lds, c, Cb, lp, tl, true, tw);
+  if (MODE != 2) fft_line<T, false, EMAX, kColEarlyTw<T>, GEN>(lds, c, Cb, lp, tl, true, tw);
   if (MODE == 0) {
     const int ky = cb * Cb + c;
     if (ky != 0) {
@@ -656,7 +658,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_co
     }
     __syncthreads();
   }
-  if (MODE != 1) fft_line<T, true, EMAX, kColEarlyTw<T>>(lds, c, Cb, lp, tl, true, tw);
+  if (MODE != 1) fft_line<T, true, EMAX, kColEarlyTw<T>, GEN>(lds, c, Cb, lp, tl, true, tw);
   slab_store(S, lds, base, n);
 }
 
@@ -701,7 +703,7 @@ __device__ __forceinline__ void c2r_finish(const P& p, const Geom& g, int row, i
 
 // Row pass: packed half spectrum -> real rows (inverse, unnormalised; the
 // round trip r2c -> c2r scales by Mx*My, which the kernels G/K absorb).
-template <typename T, int EMAX, class P>
+template <typename T, int EMAX, class P, bool GEN = false>
 __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_c2r(Geom g, LinePlan lp, const cplx<T>* __restrict__ tw,
                                              const cplx<T>* __restrict__ twp, InstArr<typename P::F> ia,
                                              int ctr_slot) {
@@ -767,19 +769,20 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_c2
     // long lines (16/32-element builds): one generic consumer loop
     cplx<T> v[EMAX];
     const typename P::Ctx cx = p.ctx();
-    fft_line_x<T, true, EMAX, false, true>(v, lds, rb, 1, lp, tl, tw);
+    fft_line_x<T, true, EMAX, false, true, true, GEN>(v, lds, rb, 1, lp, tl, tw);
     const int Rl = lp.rlast;
     const int nbl = nbutterflies(N2, Rl, tl, lp.tpl);
+    constexpr int KB = P::kBatch;
 #pragma unroll
-    for (int e0 = 0; e0 < EMAX; e0 += P::kBatch) {
-      typename P::Rec rec[P::kBatch];
+    for (int e0 = 0; e0 < EMAX; e0 += KB) {
+      typename P::Rec rec[KB];
 #pragma unroll
-      for (int k = 0; k < P::kBatch; ++k) {
+      for (int k = 0; k < KB; ++k) {
         const int e = e0 + k;
         if (e / Rl < nbl) rec[k] = p.fetch(g, row, 2 * slot_element(e, Rl, N2, tl, lp.tpl));
       }
 #pragma unroll
-      for (int k = 0; k < P::kBatch; ++k) {
+      for (int k = 0; k < KB; ++k) {
         const int e = e0 + k;
         if (e / Rl < nbl) p.take(g, row, 2 * slot_element(e, Rl, N2, tl, lp.tpl), rec[k], v[e].x, v[e].y, acc, cx);
       }
@@ -842,6 +845,7 @@ __global__ void __launch_bounds__(256) k_elem(Geom g, InstArr<T> ia, int ctr_slo
 template <typename TF, typename TT = TF>
 struct ProdCopy {  // input row from src
   using F = TF;
+  static constexpr int kBatch = kPairs;
   static constexpr int NS = 0;
   __device__ static constexpr int op(int) { return RSUM; }
   struct Rec {
@@ -1154,6 +1158,7 @@ __global__ void __launch_bounds__(1024) k_bisect(Geom g, InstArr<T> ia) {
 template <typename T>
 struct ProdDir {
   using F = T;
+  static constexpr int kBatch = kPairs;
   static constexpr int NS = 1;
   __device__ static constexpr int op(int) { return RCSUM; }
   struct Rec {
@@ -1308,6 +1313,7 @@ struct ElemPhaseB {
 template <typename T, typename TT, bool INC>
 struct ProdForce {
   using F = T;
+  static constexpr int kBatch = kPairs;
   static constexpr int NS = 1;  // sum f over comp
   __device__ static constexpr int op(int) { return RCSUM; }
   struct Rec {
@@ -1759,8 +1765,8 @@ void dev_free(cm_ctx* ctx, void* p, size_t bytes) {
 // first).  A pass of radix R has L/R butterflies dealt round-robin to tpl
 // threads; a thread holds at most ceil(L/R/tpl)*R elements, which must fit
 // the kernel's register budget emax.  Any other prime factor p gets a generic
-// pass (one thread per output element, ceil(L/tpl) <= emax; 16/32-element
-// builds only); such lines begin and end with an identity pass, so the kernels' register-ended first
+// pass (one thread per output element, ceil(L/tpl) <= emax; the 16-element
+// GEN builds only); such lines begin and end with an identity pass, so the kernels' register-ended first
 // and last passes see radix 1 (element tl + e*tpl in register slot e).
 // Prefer tpl dividing every L/R (balanced passes), then larger tpl.
 bool make_line_plan(int L, int max_tpl, int emax, LinePlan& best) {
@@ -1789,7 +1795,7 @@ bool make_line_plan(int L, int max_tpl, int emax, LinePlan& best) {
     if (rem % 2 == 0 || rem % 3 == 0 || rem % 5 == 0) return false;  // a factor the budget emax left over
     generic.push_back(rem);
   }
-  if (!generic.empty() && emax < 16) return false;  // generic passes exist in the 16/32-element builds
+  if (!generic.empty() && emax != 16) return false;  // generic passes exist in the 16-element GEN builds
   std::vector<int> seq;
   if (generic.empty()) {
     seq = unrolled;
@@ -1866,6 +1872,7 @@ struct FftExec {
   Geom g{};
   LinePlan row{}, col{}, col_rev{};  // col_rev: col with the radix order reversed
   int emax_row = 16, emax_col = 16;
+  bool gen_row = false, gen_col = false;  // a generic radix: the 16-element GEN builds run the line
   void* tw_row = nullptr;
   void* tw_col = nullptr;
   void* tw_post = nullptr;
@@ -1908,15 +1915,14 @@ struct Shape {
 };
 
 int emax_bucket(int E) { return E <= 8 ? 8 : (E <= 16 ? 16 : 32); }
-// Register build of a line plan: the smallest holding E elements, and at
-// least the 16-element one when a pass has a generic radix (the 8-element
-// builds have no generic passes).
-int plan_emax(const LinePlan& lp) {
-  int e = emax_bucket(lp.E);
+bool has_generic(const LinePlan& lp) {
   for (int s = 0; s < lp.npass; ++s)
-    if (!unrolled_radix(lp.radix(s))) e = std::max(e, 16);
-  return e;
+    if (!unrolled_radix(lp.radix(s))) return true;
+  return false;
 }
+// Register build of a line plan: the smallest holding E elements; lines with
+// a generic radix run the 16-element GEN builds.
+int plan_emax(const LinePlan& lp) { return has_generic(lp) ? 16 : emax_bucket(lp.E); }
 
 template <typename T>
 bool choose_shape(FftExec& ex, const Shape& sh) {
@@ -1967,6 +1973,8 @@ bool choose_shape(FftExec& ex, const Shape& sh) {
   ex.lds_col = lds_bytes<T>((size_t)Mx * Cb);
   ex.emax_row = plan_emax(ex.row);
   ex.emax_col = plan_emax(ex.col);
+  ex.gen_row = has_generic(ex.row);
+  ex.gen_col = has_generic(ex.col);
   ex.col_rev = ex.col;
   for (int q = 0; q < ex.col.npass; ++q) ex.col_rev.set_radix(q, ex.col.radix(ex.col.npass - 1 - q));
   ex.col_rev.rlast = ex.col_rev.radix(ex.col.npass - 1);
@@ -2052,7 +2060,10 @@ void launch_r2c(const FftExec& ex, InstArr<typename P::F> ia, int B, int slot, h
   dim3 grid(ex.g.Mx / ex.g.R, B), block(ex.row.tpl * ex.g.R);
   const cplx<T>* tw = (const cplx<T>*)ex.tw_row;
   const cplx<T>* twp = (const cplx<T>*)ex.tw_post;
-  if (ex.emax_row == 8) {
+  if (ex.gen_row) {
+    if (lds_ok((const void*)k_r2c<T, 16, P, true>, ex.lds_row))
+      hipLaunchKernelGGL((k_r2c<T, 16, P, true>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
+  } else if (ex.emax_row == 8) {
     if (lds_ok((const void*)k_r2c<T, 8, P>, ex.lds_row))
       hipLaunchKernelGGL((k_r2c<T, 8, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
   } else if (ex.emax_row == 16) {
@@ -2069,7 +2080,10 @@ void launch_c2r(const FftExec& ex, InstArr<typename P::F> ia, int B, int slot, h
   dim3 grid(ex.g.Mx / ex.g.R, B), block(ex.row.tpl * ex.g.R);
   const cplx<T>* tw = (const cplx<T>*)ex.tw_row;
   const cplx<T>* twp = (const cplx<T>*)ex.tw_post;
-  if (ex.emax_row == 8) {
+  if (ex.gen_row) {
+    if (lds_ok((const void*)k_c2r<T, 16, P, true>, ex.lds_row))
+      hipLaunchKernelGGL((k_c2r<T, 16, P, true>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
+  } else if (ex.emax_row == 8) {
     if (lds_ok((const void*)k_c2r<T, 8, P>, ex.lds_row))
       hipLaunchKernelGGL((k_c2r<T, 8, P>), grid, block, ex.lds_row, st, ex.g, ex.row, tw, twp, ia, slot);
   } else if (ex.emax_row == 16) {
@@ -2085,7 +2099,11 @@ template <typename T, int MODE, typename TF = T>
 void launch_col(const FftExec& ex, InstArr<TF> ia, int B, int useK, int skip, hipStream_t st) {
   dim3 grid(ex.g.N2 / ex.g.Cb, B), block(ex.col.tpl * ex.g.Cb);
   const cplx<T>* tw = (const cplx<T>*)ex.tw_col;
-  if (ex.emax_col == 8) {
+  if (ex.gen_col) {
+    if (lds_ok((const void*)k_col<T, 16, MODE, TF, true>, ex.lds_col))
+      hipLaunchKernelGGL((k_col<T, 16, MODE, TF, true>), grid, block, ex.lds_col, st, ex.g, ex.col, ex.col_rev, tw, ia,
+                         useK, skip);
+  } else if (ex.emax_col == 8) {
     if (lds_ok((const void*)k_col<T, 8, MODE, TF>, ex.lds_col))
       hipLaunchKernelGGL((k_col<T, 8, MODE, TF>), grid, block, ex.lds_col, st, ex.g, ex.col, ex.col_rev, tw, ia, useK,
                          skip);
