@@ -397,6 +397,21 @@ __device__ __forceinline__ int slot_element(int e, int R, int L, int tl, int tpl
   return tl + (e / R) * tpl + (e % R) * (L / R);
 }
 
+// slot_element for a run-time radix R of a first or last pass (1, 2, 3, 4, 5,
+// 8 or 16) and compile-time slots e < 32: e / R as a multiply and shift with
+// m = ceil(2^16 / R), exact while e < 2^16 / R, so the offsets are a few
+// scalar operations on uniform values instead of a division per slot.
+struct SlotMap {
+  int R, m, LR, tpl;
+  __device__ __forceinline__ SlotMap(int R_, int L, int tpl_)
+      : R(R_), m((65536 + R_ - 1) / R_), LR(L / R_), tpl(tpl_) {}
+  // line element of slot e, less the thread's lane
+  __device__ __forceinline__ int off(int e) const {
+    const int q = (e * m) >> 16;
+    return q * tpl + (e - q * R) * LR;
+  }
+};
+
 template <typename T, int R, bool INV, int EMAX>
 __device__ __forceinline__ void first_pass_regs(const cplx<T>* v, cplx<T>* lds, int base, int stride, int L,
                                                 int tl, int tpl) {
@@ -553,53 +568,92 @@ __device__ __forceinline__ void kcombine(double& s1, double& c1, double s2, doub
 // Value of a compensated pair; an infinite or NaN sum keeps its value.
 __device__ __forceinline__ double kvalue(double s, double c) { return (s - s == 0.0) ? s + c : s; }
 
-// Butterfly reduction over the lanes of one wavefront.  A block whose size is
-// not a multiple of 64 has a partial last wave: lanes past the block end do
-// not exist, so their (undefined) values are skipped.
-__device__ __forceinline__ void wave_reduce(int op, double& v, double& c) {
+// Butterfly reduction of NS slots over the lanes of one wavefront (slot i
+// with op ops[i], compile-time after inlining).  The slots are interleaved
+// step by step, so the lane exchanges of all slots are in flight together
+// instead of one slot's six dependent steps after another's.  A block whose
+// size is not a multiple of 64 has a partial last wave: lanes past the block
+// end do not exist, so their (undefined) values are skipped.
+template <int NS>
+__device__ __forceinline__ void wave_reduce(double* v, double* c, const int* ops) {
   const int lane = threadIdx.x & 63;
   const int nlanes = min(64, (int)blockDim.x - (int)(threadIdx.x & ~63u));
+  // groups of up to four slots: their exchanges overlap without raising the
+  // register peak of the kernels that reduce seven or eight slots
+  constexpr int kG = 4;
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const double o = __shfl_xor(v, off, 64);
-    if (op == RCSUM) {
-      const double oc = __shfl_xor(c, off, 64);
-      if ((lane ^ off) < nlanes) kcombine(v, c, o, oc);
-    } else if ((lane ^ off) < nlanes) {
-      v = red_apply(op, v, o);
+  for (int g0 = 0; g0 < NS; g0 += kG) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      double o[kG], oc[kG];
+#pragma unroll
+      for (int k = 0; k < kG; ++k) {
+        const int i = g0 + k;
+        if (i < NS) {
+          o[k] = __shfl_xor(v[i], off, 64);
+          oc[k] = ops[i] == RCSUM ? __shfl_xor(c[i], off, 64) : 0.0;
+        }
+      }
+      if ((lane ^ off) < nlanes) {
+#pragma unroll
+        for (int k = 0; k < kG; ++k) {
+          const int i = g0 + k;
+          if (i < NS) {
+            if (ops[i] == RCSUM)
+              kcombine(v[i], c[i], o[k], oc[k]);
+            else
+              v[i] = red_apply(ops[i], v[i], o[k]);
+          }
+        }
+      }
     }
   }
 }
 
 // Block-wide reduction of NS slots (vals) and the correction terms of the
 // compensated ones (comp); result valid in thread 0.  `scratch` must hold
-// 2 * NS * 16 doubles of shared memory (up to 16 waves).
+// 2 * NS * 16 doubles of shared memory (up to 16 waves).  Slot i is combined
+// across the waves, in wave order, by thread i (all slots at once).
 template <int NS>
 __device__ __forceinline__ void block_reduce(double* vals, double* comp, const int* ops, double* scratch) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  double v[NS], c[NS];
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
-    double v = vals[i], c = comp[i];
-    wave_reduce(ops[i], v, c);
-    if (lane == 0) {
-      scratch[i * 16 + wid] = v;
-      if (ops[i] == RCSUM) scratch[(NS + i) * 16 + wid] = c;
+    v[i] = vals[i];
+    c[i] = comp[i];
+  }
+  wave_reduce<NS>(v, c, ops);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      scratch[i * 16 + wid] = v[i];
+      scratch[(NS + i) * 16 + wid] = ops[i] == RCSUM ? c[i] : 0.0;
     }
+  }
+  __syncthreads();
+  if (threadIdx.x < NS) {
+    const int i = threadIdx.x;
+    int op = ops[0];  // ops[i] without indexing the array at run time
+#pragma unroll
+    for (int k = 1; k < NS; ++k)
+      if (i == k) op = ops[k];
+    double v = scratch[i * 16], c = scratch[(NS + i) * 16];
+    for (int w = 1; w < nw; ++w) {
+      if (op == RCSUM)
+        kcombine(v, c, scratch[i * 16 + w], scratch[(NS + i) * 16 + w]);
+      else
+        v = red_apply(op, v, scratch[i * 16 + w]);
+    }
+    scratch[i * 16] = v;
+    scratch[(NS + i) * 16] = c;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
-      double v = scratch[i * 16];
-      double c = ops[i] == RCSUM ? scratch[(NS + i) * 16] : 0.0;
-      for (int w = 1; w < nw; ++w) {
-        if (ops[i] == RCSUM)
-          kcombine(v, c, scratch[i * 16 + w], scratch[(NS + i) * 16 + w]);
-        else
-          v = red_apply(ops[i], v, scratch[i * 16 + w]);
-      }
-      vals[i] = v;
-      comp[i] = c;
+      vals[i] = scratch[i * 16];
+      comp[i] = scratch[(NS + i) * 16];
     }
   }
   __syncthreads();

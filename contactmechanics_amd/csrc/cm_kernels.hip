@@ -192,33 +192,38 @@ __device__ __forceinline__ bool reduce_blocks(double* acc, double* part, unsigne
   if (!s_last) return false;
   acquire_agent();
   double v[NS > 0 ? NS : 1], vc[NS > 0 ? NS : 1];
-  // one slot at a time (only that slot's batch of partials is live: the
-  // last block's registers count for every block of the kernel), kB blocks
-  // per thread issued together so their loads are in flight at once; plain
-  // loads, the acquire above makes every block's partials visible
-  constexpr int kB = 4;
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
-    const bool cs = ops[i] == RCSUM;
-    double vi = red_identity(ops[i]), ci = 0.0;
-    for (int b0 = threadIdx.x; b0 < nblocks; b0 += kB * blockDim.x) {
-      double w[kB], wc[kB];
+    v[i] = red_identity(ops[i]);
+    vc[i] = 0.0;
+  }
+  // every slot of kB blocks per thread loaded together (one round trip for
+  // up to kB * blockDim blocks), then combined in block order; plain loads,
+  // the acquire above makes every block's partials visible.  kB * NS stays
+  // at most 8 values (16 with corrections) in registers.
+  constexpr int kB = NS <= 2 ? 4 : (NS <= 4 ? 2 : 1);
+  for (int b0 = threadIdx.x; b0 < nblocks; b0 += kB * blockDim.x) {
+    double w[kB][NS > 0 ? NS : 1], wc[kB][NS > 0 ? NS : 1];
 #pragma unroll
-      for (int k = 0; k < kB; ++k) {
-        const int b = b0 + k * (int)blockDim.x;
-        w[k] = b < nblocks ? part[(size_t)i * nblocks + b] : red_identity(ops[i]);
-        wc[k] = (cs && b < nblocks) ? part[(size_t)(NS + i) * nblocks + b] : 0.0;
-      }
+    for (int k = 0; k < kB; ++k) {
+      const int b = b0 + k * (int)blockDim.x;
 #pragma unroll
-      for (int k = 0; k < kB; ++k) {
-        if (cs)
-          kcombine(vi, ci, w[k], wc[k]);
-        else
-          vi = red_apply(ops[i], vi, w[k]);
+      for (int i = 0; i < NS; ++i) {
+        const bool cs = ops[i] == RCSUM;
+        w[k][i] = b < nblocks ? part[(size_t)i * nblocks + b] : red_identity(ops[i]);
+        wc[k][i] = (cs && b < nblocks) ? part[(size_t)(NS + i) * nblocks + b] : 0.0;
       }
     }
-    v[i] = vi;
-    vc[i] = ci;
+#pragma unroll
+    for (int k = 0; k < kB; ++k) {
+#pragma unroll
+      for (int i = 0; i < NS; ++i) {
+        if (ops[i] == RCSUM)
+          kcombine(v[i], vc[i], w[k][i], wc[k][i]);
+        else
+          v[i] = red_apply(ops[i], v[i], w[k][i]);
+      }
+    }
   }
   block_reduce<NS>(v, vc, ops, scr);
   if (threadIdx.x == 0) {
@@ -396,6 +401,19 @@ struct RowBlockMap {
   }
 };
 
+// Row block of a spectrum-reading row pass, XCD-contiguous: the hardware hands
+// the workgroups of a launch to its 8 XCDs in turn (workgroup b to XCD b % 8),
+// each XCD with its own L2.  Neighbouring row blocks read the two halves of
+// the same 128-byte lines of the column-blocked spectrum (runs of Cb x R
+// values), so this map gives each XCD one contiguous range of row blocks: the
+// partner of a line then reads it from the same L2 instead of fetching the
+// whole line again.  A bijection of [0, n) for any n.
+__device__ __forceinline__ int xcd_row_block(int b, int n) {
+  constexpr int kXcd = 8;
+  const int q = n / kXcd, rem = n - q * kXcd, x = b % kXcd;
+  return x * q + min(x, rem) + b / kXcd;
+}
+
 // The last block's scalar logic gets a copy of the reduced slots, so the
 // accumulator array itself never has its address taken and stays in registers
 // (the functors' finalize is not necessarily inlined).
@@ -442,20 +460,21 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_r2
   cplx<T> v[EMAX];
   {
     const int R0 = lp.radix(0);
-    const int nb0 = nbutterflies(N2, R0, tl, lp.tpl);
+    const int nv0 = nbutterflies(N2, R0, tl, lp.tpl) * R0;  // slots holding a pair
+    const SlotMap sm(R0, N2, lp.tpl);
     constexpr int KB = P::kBatch;
-    if (nb0 * R0 >= EMAX) {
+    if (nv0 >= EMAX) {
       // every register slot holds a point pair: no per-slot branch between
       // the loads of a batch, so they all issue before the first is used
 #pragma unroll
       for (int e0 = 0; e0 < EMAX; e0 += KB) {
         typename P::Rec rec[KB];
 #pragma unroll
-        for (int k = 0; k < KB; ++k) rec[k] = p.fetch(g, row, 2 * slot_element(e0 + k, R0, N2, tl, lp.tpl));
+        for (int k = 0; k < KB; ++k) rec[k] = p.fetch(g, row, 2 * (tl + sm.off(e0 + k)));
 #pragma unroll
         for (int k = 0; k < KB; ++k) {
           T a, b;
-          p.make(g, row, 2 * slot_element(e0 + k, R0, N2, tl, lp.tpl), rec[k], a, b, acc);
+          p.make(g, row, 2 * (tl + sm.off(e0 + k)), rec[k], a, b, acc);
           v[e0 + k] = mk(a, b);
         }
       }
@@ -466,14 +485,14 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_r2
 #pragma unroll
         for (int k = 0; k < KB; ++k) {
           const int e = e0 + k;
-          if (e / R0 < nb0) rec[k] = p.fetch(g, row, 2 * slot_element(e, R0, N2, tl, lp.tpl));
+          if (e < nv0) rec[k] = p.fetch(g, row, 2 * (tl + sm.off(e)));
         }
 #pragma unroll
         for (int k = 0; k < KB; ++k) {
           const int e = e0 + k;
-          if (e / R0 < nb0) {
+          if (e < nv0) {
             T a, b;
-            p.make(g, row, 2 * slot_element(e, R0, N2, tl, lp.tpl), rec[k], a, b, acc);
+            p.make(g, row, 2 * (tl + sm.off(e)), rec[k], a, b, acc);
             v[e] = mk(a, b);
           }
         }
@@ -514,6 +533,19 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_r2
     if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
       finalize_copy(p, acc);
   }
+}
+
+// A column-pass thread's first-pass register slots straight from the
+// column-blocked spectrum (column c of the block at base): every slot loads,
+// slots past the thread's butterflies from an in-range element whose value
+// nothing reads, so the loads carry no branch and all issue before the first
+// is waited for.
+template <typename T, int EMAX>
+__device__ __forceinline__ void col_load_slots(cplx<T>* v, const cplx<T>* S, size_t base, int Cb, int c, int Mx,
+                                               int tl, const SlotMap& sm) {
+  const GView<T> Sg(S);
+#pragma unroll
+  for (int e = 0; e < EMAX; ++e) v[e] = Sg[base + (size_t)min(tl + sm.off(e), Mx - 1) * Cb + c];
 }
 
 // Column kernels load twiddles early (before each pass's barrier) only in
@@ -559,20 +591,20 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_co
     double acc[2] = {0.0, 0.0};  // compensated sum
     const int c = threadIdx.x % Cb, tl = threadIdx.x / Cb;
     if (cb != 0) {
-      const int R0 = lp.radix(0), Rl = lp.rlast;
-      const int nb0 = nbutterflies(Mx, R0, tl, lp.tpl), nbl = nbutterflies(Mx, Rl, tl, lp.tpl);
+      const int nvl = nbutterflies(Mx, lp.rlast, tl, lp.tpl) * lp.rlast;
+      const SlotMap sm0(lp.radix(0), Mx, lp.tpl), sml(lp.rlast, Mx, lp.tpl);
       cplx<T> v[EMAX];
-      const GView<T> Sg(S);
-#pragma unroll
-      for (int e = 0; e < EMAX; ++e)
-        if (e / R0 < nb0) v[e] = Sg[base + (size_t)slot_element(e, R0, Mx, tl, lp.tpl) * Cb + c];
+      col_load_slots<T, EMAX>(v, S, base, Cb, c, Mx, tl, sm0);
       fft_line_x<T, false, EMAX, true, true, kColEarlyTw<T>, GEN>(v, lds, c, Cb, lp, tl, tw);
       const auto Gk = (const __attribute__((address_space(1))) T*)(G + (size_t)(cb * Cb + c) * Mx);
+      T gk[EMAX];
+#pragma unroll
+      for (int e = 0; e < EMAX; ++e) gk[e] = Gk[min(tl + sml.off(e), Mx - 1)];
 #pragma unroll
       for (int e = 0; e < EMAX; ++e)
-        if (e / Rl < nbl) {
+        if (e < nvl) {
           const double a = (double)v[e].x, b = (double)v[e].y;
-          kadd(acc[0], acc[1], 2.0 * (double)Gk[slot_element(e, Rl, Mx, tl, lp.tpl)] * (a * a + b * b));
+          kadd(acc[0], acc[1], 2.0 * (double)gk[e] * (a * a + b * b));
         }
     } else {
       slab_load(lds, S, base, n);
@@ -607,26 +639,40 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_co
     // forward pass -> x G in registers -> first inverse pass (same radix, same
     // slots), last inverse pass -> HBM.  Two LDS round trips fewer each way.
     const int c = threadIdx.x % Cb, tl = threadIdx.x / Cb;
-    const int R0 = lp.radix(0), Rl = lp.rlast;
-    const int nb0 = nbutterflies(Mx, R0, tl, lp.tpl), nbl = nbutterflies(Mx, Rl, tl, lp.tpl);
+    const int R0 = lp.radix(0);
+    const int nv0 = nbutterflies(Mx, R0, tl, lp.tpl) * R0;
+    const SlotMap sm0(R0, Mx, lp.tpl), sml(lp.rlast, Mx, lp.tpl);
     cplx<T> v[EMAX];
-    const GView<T> Sg(S);
-#pragma unroll
-    for (int e = 0; e < EMAX; ++e)
-      if (e / R0 < nb0) v[e] = Sg[base + (size_t)slot_element(e, R0, Mx, tl, lp.tpl) * Cb + c];
+    col_load_slots<T, EMAX>(v, S, base, Cb, c, Mx, tl, sm0);
     fft_line_x<T, false, EMAX, true, true, kColEarlyTw<T>, GEN>(v, lds, c, Cb, lp, tl, tw);
     __syncthreads();
     const auto Gk = (const __attribute__((address_space(1))) T*)(G + (size_t)(cb * Cb + c) * Mx);
+    {
+      // (slots past the last pass's butterflies read an in-range G and
+      // carry values nothing reads)
+      T gk[EMAX];
 #pragma unroll
-    for (int e = 0; e < EMAX; ++e)
-      if (e / Rl < nbl) v[e] = v[e] * Gk[slot_element(e, Rl, Mx, tl, lp.tpl)];
+      for (int e = 0; e < EMAX; ++e) gk[e] = Gk[min(tl + sml.off(e), Mx - 1)];
+#pragma unroll
+      for (int e = 0; e < EMAX; ++e) v[e] = v[e] * gk[e];
+    }
     // rv: the plan with its radices reversed, so the inverse transform's first
     // pass takes the forward transform's last-pass register slots as they are
     fft_line_x<T, true, EMAX, true, true, kColEarlyTw<T>, GEN>(v, lds, c, Cb, rv, tl, tw);
     const GViewW<T> Sw(S);
+    // the store addresses are those of the loads: an opaque copy of the lane
+    // index makes the compiler form them again here instead of keeping the
+    // load addresses live (spilled) through both transforms
+    int tls = tl;
+    asm volatile("" : "+v"(tls));
+    if (nv0 >= EMAX) {
 #pragma unroll
-    for (int e = 0; e < EMAX; ++e)
-      if (e / R0 < nb0) Sw.put(base + (size_t)slot_element(e, R0, Mx, tl, lp.tpl) * Cb + c, v[e]);
+      for (int e = 0; e < EMAX; ++e) Sw.put(base + (size_t)(tls + sm0.off(e)) * Cb + c, v[e]);
+    } else {
+#pragma unroll
+      for (int e = 0; e < EMAX; ++e)
+        if (e < nv0) Sw.put(base + (size_t)(tls + sm0.off(e)) * Cb + c, v[e]);
+    }
     return;
   }
   slab_load(lds, S, base, n);
@@ -715,7 +761,8 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_c2
   const int N2 = g.N2;
   const cplx<T>* S = spec_of<T>(in);
   const int tot = g.R * N2;
-  const RowBlockMap map(g, blockIdx.x);
+  const int blk = xcd_row_block(blockIdx.x, gridDim.x);
+  const RowBlockMap map(g, blk);
   const GView<T> Sg(S);
   for (int i0 = threadIdx.x; i0 < tot; i0 += kCopyU * (int)blockDim.x) {
     // (out-of-range slots repeat the last element, as in slab_load)
@@ -731,7 +778,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_c2
   }
   __syncthreads();
   const int r = threadIdx.x / lp.tpl, tl = threadIdx.x - r * lp.tpl;
-  const int row = blockIdx.x * g.R + r;
+  const int row = blk * g.R + r;
   const int rb = r * N2;
   auto L = [&](int n) -> cplx<T>& { return lds[lpad<T>(rb + n)]; };
   for (int k = tl; k <= N2 / 2; k += lp.tpl) {
@@ -770,8 +817,8 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_c2
     cplx<T> v[EMAX];
     const typename P::Ctx cx = p.ctx();
     fft_line_x<T, true, EMAX, false, true, true, GEN>(v, lds, rb, 1, lp, tl, tw);
-    const int Rl = lp.rlast;
-    const int nbl = nbutterflies(N2, Rl, tl, lp.tpl);
+    const int nvl = nbutterflies(N2, lp.rlast, tl, lp.tpl) * lp.rlast;
+    const SlotMap sml(lp.rlast, N2, lp.tpl);
     constexpr int KB = P::kBatch;
 #pragma unroll
     for (int e0 = 0; e0 < EMAX; e0 += KB) {
@@ -779,17 +826,17 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_c2
 #pragma unroll
       for (int k = 0; k < KB; ++k) {
         const int e = e0 + k;
-        if (e / Rl < nbl) rec[k] = p.fetch(g, row, 2 * slot_element(e, Rl, N2, tl, lp.tpl));
+        if (e < nvl) rec[k] = p.fetch(g, row, 2 * (tl + sml.off(e)));
       }
 #pragma unroll
       for (int k = 0; k < KB; ++k) {
         const int e = e0 + k;
-        if (e / Rl < nbl) p.take(g, row, 2 * slot_element(e, Rl, N2, tl, lp.tpl), rec[k], v[e].x, v[e].y, acc, cx);
+        if (e < nvl) p.take(g, row, 2 * (tl + sml.off(e)), rec[k], v[e].x, v[e].y, acc, cx);
       }
     }
   }
   if constexpr (P::NS > 0) {
-    if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
+    if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blk) && threadIdx.x == 0)
       finalize_copy(p, acc);
   }
 }
@@ -797,7 +844,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_c2
 // Element-wise kernel with fused reductions: blocks stride over rows, threads
 // over point pairs of a row (no index division, coalesced pair loads).
 template <typename T, class P>
-__global__ void __launch_bounds__(256) k_elem(Geom g, InstArr<T> ia, int ctr_slot) {
+__global__ void __launch_bounds__(256, 4) k_elem(Geom g, InstArr<T> ia, int ctr_slot) {
   const Inst<T>& in = ia.get(blockIdx.y);
   P p(in);
   if (p.skip()) return;
@@ -1398,10 +1445,10 @@ struct ConsDisp {
   // iteration (its contact set comes from the mask bits the force pass wrote
   // from the final forces)
   static constexpr int NS = 8;
-  // output pairs per batch: two in FP64 (the record by value and the
-  // transform's registers leave no room for four under 128 VGPRs, four waves
-  // per SIMD), four in FP32 (88 VGPRs with two)
-  static constexpr int kBatch = sizeof(T) == 4 ? 4 : 2;
+  // output pairs per batch: two with FP64 transforms (the record by value
+  // and the transform's registers leave no room for four under 128 VGPRs,
+  // four waves per SIMD), four with FP32 transforms (88 VGPRs with two)
+  static constexpr int kBatch = sizeof(TT) == 4 ? 4 : 2;
   __device__ static constexpr int op(int i) { return i < 2 ? RMAX : phase_a_op(i - 2); }
   struct Rec {
     P2<double> w;
