@@ -568,23 +568,95 @@ __device__ __forceinline__ void kcombine(double& s1, double& c1, double s2, doub
 // Value of a compensated pair; an infinite or NaN sum keeps its value.
 __device__ __forceinline__ double kvalue(double s, double c) { return (s - s == 0.0) ? s + c : s; }
 
+// Value of the partner lane of step OFF of an ascending butterfly (OFF = 1,
+// 2, 4, 8, 16, 32) over a full wavefront, in the vector ALU instead of the
+// LDS crossbar: DPP quad permutes for 1 and 2 (lane ^ OFF), DPP row
+// half-mirror and mirror for 4 and 8 (lane 7 - i, 15 - i of its group: a lane
+// of the partner group, which holds one value once the smaller steps are
+// done), permlane16/32 swaps for 16 and 32 (lane ^ OFF).
+template <int OFF>
+__device__ __forceinline__ double lane_partner(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  int nlo, nhi;
+  if constexpr (OFF == 1 || OFF == 2 || OFF == 4 || OFF == 8) {
+    constexpr int ctrl = OFF == 1 ? 0xB1 : (OFF == 2 ? 0x4E : (OFF == 4 ? 0x141 : 0x140));
+    nlo = __builtin_amdgcn_update_dpp(0, lo, ctrl, 0xF, 0xF, false);
+    nhi = __builtin_amdgcn_update_dpp(0, hi, ctrl, 0xF, 0xF, false);
+  } else if constexpr (OFF == 16) {
+    // odd rows of the first result hold the even rows' values, even rows of
+    // the second the odd rows'
+    const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    const bool odd = (threadIdx.x & 16u) != 0;
+    nlo = odd ? a[0] : a[1];
+    nhi = odd ? b[0] : b[1];
+  } else {
+    static_assert(OFF == 32, "butterfly steps are 1, 2, 4, 8, 16, 32");
+    const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    const bool up = (threadIdx.x & 32u) != 0;
+    nlo = up ? a[0] : a[1];
+    nhi = up ? b[0] : b[1];
+  }
+  return __hiloint2double(nhi, nlo);
+}
+
+// One step of the full-wave butterfly for the slots [G0, G0 + kG) of NS.
+template <int NS, int OFF, int G0, int kG>
+__device__ __forceinline__ void wave_step(double* v, double* c, const int* ops) {
+  double o[kG], oc[kG];
+#pragma unroll
+  for (int k = 0; k < kG; ++k) {
+    const int i = G0 + k;
+    if (i < NS) {
+      o[k] = lane_partner<OFF>(v[i]);
+      oc[k] = ops[i] == RCSUM ? lane_partner<OFF>(c[i]) : 0.0;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kG; ++k) {
+    const int i = G0 + k;
+    if (i < NS) {
+      if (ops[i] == RCSUM)
+        kcombine(v[i], c[i], o[k], oc[k]);
+      else
+        v[i] = red_apply(ops[i], v[i], o[k]);
+    }
+  }
+}
+template <int NS, int G0, int kG>
+__device__ __forceinline__ void wave_steps(double* v, double* c, const int* ops) {
+  wave_step<NS, 1, G0, kG>(v, c, ops);
+  wave_step<NS, 2, G0, kG>(v, c, ops);
+  wave_step<NS, 4, G0, kG>(v, c, ops);
+  wave_step<NS, 8, G0, kG>(v, c, ops);
+  wave_step<NS, 16, G0, kG>(v, c, ops);
+  wave_step<NS, 32, G0, kG>(v, c, ops);
+  if constexpr (G0 + kG < NS) wave_steps<NS, G0 + kG, kG>(v, c, ops);
+}
+
 // Butterfly reduction of NS slots over the lanes of one wavefront (slot i
-// with op ops[i], compile-time after inlining).  The slots are interleaved
-// step by step, so the lane exchanges of all slots are in flight together
-// instead of one slot's six dependent steps after another's.  A block whose
-// size is not a multiple of 64 has a partial last wave: lanes past the block
-// end do not exist, so their (undefined) values are skipped.
+// with op ops[i], compile-time after inlining); every lane ends with the
+// result.  Slots go in groups of up to four, interleaved step by step, so
+// their exchanges overlap without raising the register peak of the kernels
+// that reduce seven or eight slots.  The combinations are commutative
+// (TwoSum's error term is exact in either order), so both lanes of a pair
+// form the same value.  A block whose size is not a multiple of 64 has a
+// partial last wave: lanes past the block end do not exist, so that wave
+// exchanges through the LDS crossbar and skips them.
 template <int NS>
 __device__ __forceinline__ void wave_reduce(double* v, double* c, const int* ops) {
   const int lane = threadIdx.x & 63;
   const int nlanes = min(64, (int)blockDim.x - (int)(threadIdx.x & ~63u));
-  // groups of up to four slots: their exchanges overlap without raising the
-  // register peak of the kernels that reduce seven or eight slots
   constexpr int kG = 4;
+  if (nlanes == 64) {
+    wave_steps<NS, 0, kG>(v, c, ops);
+    return;
+  }
 #pragma unroll
   for (int g0 = 0; g0 < NS; g0 += kG) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
+    for (int off = 1; off < 64; off <<= 1) {
       double o[kG], oc[kG];
 #pragma unroll
       for (int k = 0; k < kG; ++k) {
@@ -634,10 +706,16 @@ __device__ __forceinline__ void block_reduce(double* vals, double* comp, const i
   __syncthreads();
   if (threadIdx.x < NS) {
     const int i = threadIdx.x;
-    int op = ops[0];  // ops[i] without indexing the array at run time
+    // ops[i] from bit masks of the (compile-time) ops, so no private copy of
+    // the array is indexed at run time
+    unsigned m_cs = 0, m_max = 0, m_min = 0;
 #pragma unroll
-    for (int k = 1; k < NS; ++k)
-      if (i == k) op = ops[k];
+    for (int k = 0; k < NS; ++k) {
+      m_cs |= (ops[k] == RCSUM ? 1u : 0u) << k;
+      m_max |= (ops[k] == RMAX ? 1u : 0u) << k;
+      m_min |= (ops[k] == RMIN ? 1u : 0u) << k;
+    }
+    const int op = ((m_cs >> i) & 1u) ? RCSUM : ((m_max >> i) & 1u) ? RMAX : ((m_min >> i) & 1u) ? RMIN : RSUM;
     double v = scratch[i * 16], c = scratch[(NS + i) * 16];
     for (int w = 1; w < nw; ++w) {
       if (op == RCSUM)

@@ -414,15 +414,47 @@ __device__ __forceinline__ int xcd_row_block(int b, int n) {
   return x * q + min(x, rem) + b / kXcd;
 }
 
-// The last block's scalar logic gets a copy of the reduced slots, so the
-// accumulator array itself never has its address taken and stays in registers
-// (the functors' finalize is not necessarily inlined).
+// The last block's scalar logic (thread 0) on a copy of the scalar record in
+// LDS: the block loads the record with one load per thread and stores it
+// back the same way, so the logic's dependent reads and writes of many fields
+// cost LDS latency instead of one memory round trip each.  Called by every
+// thread of the last block; when it runs, every other block of the kernel has
+// read the record already (each reads it when it starts, before its ticket)
+// and no kernel behind this one has started.
+template <typename F>
+__device__ __forceinline__ void with_scal_in_lds(Scal* gs, F&& logic) {
+  __shared__ Scal ls;
+  constexpr int nw = (int)(sizeof(Scal) / sizeof(unsigned));
+  static_assert(sizeof(Scal) % sizeof(unsigned) == 0, "the record copies as 32-bit words");
+  const auto src = (const __attribute__((address_space(1))) unsigned*)gs;
+  unsigned* dst = reinterpret_cast<unsigned*>(&ls);
+  for (int i = threadIdx.x; i < nw; i += blockDim.x) dst[i] = src[i];
+  __syncthreads();
+  if (threadIdx.x == 0) logic(&ls);
+  __syncthreads();
+  const auto out = (__attribute__((address_space(1))) unsigned*)gs;
+  for (int i = threadIdx.x; i < nw; i += blockDim.x) out[i] = dst[i];
+}
+// Functors whose finalize runs the long scalar logic (P::kScalInLds) use the
+// LDS copy; the others write a field or two directly.
+template <class P, class = void>
+struct scal_in_lds : std::false_type {};
 template <class P>
-__device__ __forceinline__ void finalize_copy(const P& p, const double* acc) {
+struct scal_in_lds<P, std::void_t<decltype(P::kScalInLds)>> : std::bool_constant<P::kScalInLds> {};
+
+// The last block's finalize on the reduced slots (a copy, so the accumulator
+// array itself never has its address taken and stays in registers: the
+// functors' finalize is not necessarily inlined).  Every thread of the last
+// block calls it.
+template <class P>
+__device__ __forceinline__ void finalize_block(const P& p, const double* acc, Scal* gs) {
   double fin[P::NS > 0 ? P::NS : 1];
 #pragma unroll
   for (int i = 0; i < P::NS; ++i) fin[i] = P::op(i) == RCSUM ? kvalue(acc[i], acc[P::NS + i]) : acc[i];
-  p.finalize(fin);
+  if constexpr (scal_in_lds<P>::value)
+    with_scal_in_lds(gs, [&](Scal* s) { p.finalize(fin, s); });
+  else if (threadIdx.x == 0)
+    p.finalize(fin, gs);
 }
 
 // Largest block of an FFT kernel: 512 threads.  The lean 8-element builds
@@ -530,8 +562,8 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_r2
     }
   }
   if constexpr (P::NS > 0) {
-    if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
-      finalize_copy(p, acc);
+    if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blockIdx.x))
+      finalize_block(p, acc, in.sc);
   }
 }
 
@@ -763,6 +795,13 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_c2
   const int tot = g.R * N2;
   const int blk = xcd_row_block(blockIdx.x, gridDim.x);
   const RowBlockMap map(g, blk);
+  const int r = threadIdx.x / lp.tpl, tl = threadIdx.x - r * lp.tpl;
+  // the merge pass's first twiddles, loaded with the slab: the merge runs
+  // after the slab's barrier, so their latency overlaps the slab's
+  constexpr int kTwPre = 4;
+  cplx<T> twk[kTwPre];
+#pragma unroll
+  for (int j = 0; j < kTwPre; ++j) twk[j] = twp[min(tl + j * lp.tpl, N2 / 2)];
   const GView<T> Sg(S);
   for (int i0 = threadIdx.x; i0 < tot; i0 += kCopyU * (int)blockDim.x) {
     // (out-of-range slots repeat the last element, as in slab_load)
@@ -777,11 +816,10 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_c2
     for (int k = 0; k < kCopyU; ++k) lds[lpad<T>(map.lds(min(i0 + k * (int)blockDim.x, tot - 1)))] = mk(tx[k], ty[k]);
   }
   __syncthreads();
-  const int r = threadIdx.x / lp.tpl, tl = threadIdx.x - r * lp.tpl;
   const int row = blk * g.R + r;
   const int rb = r * N2;
   auto L = [&](int n) -> cplx<T>& { return lds[lpad<T>(rb + n)]; };
-  for (int k = tl; k <= N2 / 2; k += lp.tpl) {
+  auto merge = [&](int k, const cplx<T>& w) {
     if (k == 0) {
       cplx<T> z = L(0);  // (X[0], X[N2])
       L(0) = mk(z.x + z.y, z.x - z.y);
@@ -789,11 +827,17 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_c2
       const int km = N2 - k;
       cplx<T> xk = L(k), xm = L(km);
       cplx<T> E = xk + conj(xm);
-      cplx<T> O = cmulc(xk - conj(xm), twp[k]);
+      cplx<T> O = cmulc(xk - conj(xm), w);
       L(k) = mk(E.x - O.y, E.y + O.x);            // E + iO
       if (km != k) L(km) = mk(E.x + O.y, O.x - E.y);  // conj(E) + i conj(O)
     }
+  };
+#pragma unroll
+  for (int j = 0; j < kTwPre; ++j) {
+    const int k = tl + j * lp.tpl;
+    if (k <= N2 / 2) merge(k, twk[j]);
   }
+  for (int k = tl + kTwPre * lp.tpl; k <= N2 / 2; k += lp.tpl) merge(k, twp[k]);
   __syncthreads();
   double acc[P::NS > 0 ? 2 * P::NS : 1];  // slot values, then corrections of compensated sums
 #pragma unroll
@@ -836,8 +880,8 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_c2
     }
   }
   if constexpr (P::NS > 0) {
-    if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blk) && threadIdx.x == 0)
-      finalize_copy(p, acc);
+    if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blk))
+      finalize_block(p, acc, in.sc);
   }
 }
 
@@ -880,8 +924,8 @@ __global__ void __launch_bounds__(256, 4) k_elem(Geom g, InstArr<T> ia, int ctr_
     }
   }
   if constexpr (P::NS > 0) {
-    if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blockIdx.x) && threadIdx.x == 0)
-      finalize_copy(p, acc);
+    if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blockIdx.x))
+      finalize_block(p, acc, in.sc);
   }
 }
 
@@ -906,7 +950,7 @@ struct ProdCopy {  // input row from src
     a = (TT)r.v.a;
     b = (TT)r.v.b;
   }
-  __device__ void finalize(double*) const {}
+  __device__ void finalize(double*, Scal*) const {}
 };
 
 template <typename TF, typename TT = TF>
@@ -925,7 +969,7 @@ struct ConsCopy {  // output row to dst
   __device__ void take(const Geom& g, int row, int col, const Rec&, TT a, TT b, double*, const Ctx&) const {
     st2(in.dst + (size_t)row * g.My + col, (TF)a, (TF)b);
   }
-  __device__ void finalize(double*) const {}
+  __device__ void finalize(double*, Scal*) const {}
 };
 
 // ---------------------------------------------------------------------------
@@ -965,8 +1009,7 @@ struct ElemInit {
     if (r.h.c0 || r.h.c1) st2(in.u + (size_t)row * g.My + col, u0, u1);
   }
   // the host stores the full grid size M in n_pad; subtract the comp count
-  __device__ void finalize(double* acc) const {
-    Scal* s = in.sc;
+  __device__ void finalize(double* acc, Scal* s) const {
     s->max_h = acc[0];
     s->n_pad = s->n_pad - acc[1];
   }
@@ -992,7 +1035,7 @@ struct ConsInitForce {
   __device__ void take(const Geom& g, int row, int col, const Rec& r, TT a, TT b, double*, const Ctx&) const {
     st2(in.fm + (size_t)row * g.My + col, r.c.c0 ? (double)a : 0.0, r.c.c1 ? (double)b : 0.0);
   }
-  __device__ void finalize(double*) const {}
+  __device__ void finalize(double*, Scal*) const {}
 };
 
 // The initial forces of the solve with f[pad] = 0 after u =
@@ -1025,7 +1068,7 @@ struct ElemZeroPad {
       st2(fp, r.c.c0 ? (double)r.f.a : 0.0, r.c.c1 ? (double)r.f.b : 0.0);
     }
   }
-  __device__ void finalize(double*) const {}
+  __device__ void finalize(double*, Scal*) const {}
 };
 
 // ---------------------------------------------------------------------------
@@ -1129,8 +1172,8 @@ struct ElemPhaseA {
     // FP64 plans write only where the gap differs from u (w aliases u)
     if (!std::is_same<T, double>::value || r.h.c0 || r.h.c1) st2(in.w + p, w0, w1);
   }
-  __device__ void finalize(double* a) const {
-    if (!in.sc->done) phase_a_logic(in.sc, a);
+  __device__ void finalize(double* a, Scal* s) const {
+    if (!s->done) phase_a_logic(s, a);
   }
 };
 
@@ -1239,7 +1282,7 @@ struct ProdDir {
     b = one(s, r.cp.c1, r.cm.b() & kNext, r.w.b, r.t.b, acc);
     st2(in.t + (size_t)row * g.My + col, a, b);
   }
-  __device__ void finalize(double* a) const { in.sc->Scgt = a[0]; }
+  __device__ void finalize(double* a, Scal* s) const { s->Scgt = a[0]; }
 };
 
 // Step length from sum_c r t (:284-294); r = evaluate_disp(t).
@@ -1269,6 +1312,7 @@ template <typename T>
 struct ElemPhaseB {
   using F = T;
   static constexpr int NS = 7;  // n_tensile, max_tensile, pad_max, A_fl, min_fl, sum_f2, nc
+  static constexpr bool kScalInLds = true;  // projection and step-kind logic
   __device__ static constexpr int op(int i) {
     return (i == 1 || i == 2) ? RMAX : (i == 4 ? RMIN : (i == 5 ? RCSUM : RSUM));
   }
@@ -1319,8 +1363,7 @@ struct ElemPhaseB {
     const uint8_t c1 = one(s, r.cp.c1, r.cm.b(), r.f.b, (double)r.t.b, r.w.b, acc);
     if (s->mode == MODE_MIX) stm(in.cm + (size_t)row * g.My + col, c0, c1);
   }
-  __device__ void finalize(double* a) const {
-    Scal* s = in.sc;
+  __device__ void finalize(double* a, Scal* s) const {
     s->pad_pres = s->n_pad > 0 ? a[2] : 0.0;             // :344-346
     s->max_pres = a[0] > 0 ? a[1] : 0.0;                 // :350-352
     s->A_fl = a[3];
@@ -1423,7 +1466,7 @@ struct ProdForce {
       b = (TT)f1;
     }
   }
-  __device__ void finalize(double* a) const { in.sc->total_force = -a[0]; }
+  __device__ void finalize(double* a, Scal* s) const { s->total_force = -a[0]; }
 };
 
 // u_new = evaluate_disp(f) consumed on the fly: maxdu, max_pen and the
@@ -1449,6 +1492,7 @@ struct ConsDisp {
   // and the transform's registers leave no room for four under 128 VGPRs,
   // four waves per SIMD), four with FP32 transforms (88 VGPRs with two)
   static constexpr int kBatch = sizeof(TT) == 4 ? 4 : 2;
+  static constexpr bool kScalInLds = true;  // convergence tests and the next iteration's start
   __device__ static constexpr int op(int i) { return i < 2 ? RMAX : phase_a_op(i - 2); }
   struct Rec {
     P2<double> w;
@@ -1510,8 +1554,7 @@ struct ConsDisp {
     one(cx, r.h.c1, r.cm.b(), r.w.b, r.h.h1, b, w1, acc);
     st2(in.w + (size_t)row * g.My + col, w0, w1);
   }
-  __device__ void finalize(double* a) const {
-    Scal* s = in.sc;
+  __device__ void finalize(double* a, Scal* s) const {
     s->it += 1;  // iterations completed (:235-236)
     s->nfev += 1;
     s->maxdu = a[0];
@@ -1591,7 +1634,7 @@ struct ElemFinish {
     if (row < g.nx && col < g.ny) kadd(acc[0], acc[NS], r.f.a * u0);
     if (row < g.nx && col + 1 < g.ny) kadd(acc[0], acc[NS], r.f.b * u1);
   }
-  __device__ void finalize(double* a) const { in.sc->fun = -a[0] / 2; }
+  __device__ void finalize(double* a, Scal* s) const { s->fun = -a[0] / 2; }
 };
 
 // ---------------------------------------------------------------------------

@@ -78,8 +78,74 @@ V shfl_xor(V v, int off, int width) {
   (void)width;
   return r;
 }
+// The value v of lane src of this thread's wavefront (every thread of the
+// block calls it, as with a wave-wide instruction).
+template <typename V>
+V lane_read(V v, unsigned src_lane) {
+  Fiber& m = me();
+  std::memcpy(&m.shfl, &v, sizeof(V));
+  barrier();
+  const unsigned src = (m.tid & ~63u) + src_lane;
+  V r;
+  if (src >= blk().n) {
+    const double nan = std::nan("");
+    std::memcpy(&r, &nan, sizeof(V) < sizeof(double) ? sizeof(V) : sizeof(double));
+  } else {
+    std::memcpy(&r, &blk().f[src].shfl, sizeof(V));
+  }
+  barrier();
+  return r;
+}
+struct U2 {
+  int v[2];
+  int operator[](int i) const { return v[i]; }
+};
 void run_block(std::function<void()> body, unsigned nthreads);
 }  // namespace emu
+
+// Cross-lane builtins of gfx950 used by the wave reductions, as the ISA
+// documents them: DPP quad permutes and row mirrors (row_mask and bank_mask
+// all set), and the permlane16/32 swaps of odd rows of the first operand
+// with even rows of the second.
+inline int __builtin_amdgcn_update_dpp(int, int src, int ctrl, int, int, bool) {
+  const unsigned i = emu::me().tid & 63u;
+  unsigned s;
+  if (ctrl <= 0xFF) s = (i & ~3u) + ((ctrl >> (2 * (i & 3u))) & 3u);
+  else if (ctrl == 0x140) s = (i & ~15u) + (15u - (i & 15u));
+  else if (ctrl == 0x141) s = (i & ~7u) + (7u - (i & 7u));
+  else { fprintf(stderr, "emu: DPP control %#x not emulated\n", ctrl); abort(); }
+  return emu::lane_read(src, s);
+}
+inline emu::U2 __builtin_amdgcn_permlane16_swap(int old, int src, bool, bool) {
+  const unsigned i = emu::me().tid & 63u;
+  const int src_lo = emu::lane_read(src, i >= 16 ? i - 16 : 0);   // src of the row below
+  const int old_hi = emu::lane_read(old, i + 16 < 64 ? i + 16 : 63);  // old of the row above
+  const bool odd = (i & 16u) != 0;
+  return emu::U2{{odd ? src_lo : old, odd ? src : old_hi}};
+}
+inline emu::U2 __builtin_amdgcn_permlane32_swap(int old, int src, bool, bool) {
+  const unsigned i = emu::me().tid & 63u;
+  const int src_lo = emu::lane_read(src, i >= 32 ? i - 32 : 0);
+  const int old_hi = emu::lane_read(old, i + 32 < 64 ? i + 32 : 63);
+  const bool up = (i & 32u) != 0;
+  return emu::U2{{up ? src_lo : old, up ? src : old_hi}};
+}
+inline int __double2loint(double d) {
+  long long b;
+  std::memcpy(&b, &d, 8);
+  return (int)(unsigned)(b & 0xffffffffLL);
+}
+inline int __double2hiint(double d) {
+  long long b;
+  std::memcpy(&b, &d, 8);
+  return (int)(b >> 32);
+}
+inline double __hiloint2double(int hi, int lo) {
+  const unsigned long long b = ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+  double d;
+  std::memcpy(&d, &b, 8);
+  return d;
+}
 
 struct EmuIdx {
   unsigned x, y, z;
