@@ -9,6 +9,8 @@ later calls apply the cached shape (``cm_plan_set_shape``) without timing.
 import json
 import os
 
+from ._native import CmError
+
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "contactmechanics_amd", "autotune.json")
 
 
@@ -38,8 +40,11 @@ def tune(substrate, cache_path=DEFAULT_CACHE, reps=10, force=False):
             cache = {}
     if not force and key in cache:
         c = cache[key]
-        plan.set_shape(c["Cb"], c["R"], c["erow"], c["ecol"])
-        return dict(c, cached=True)
+        try:
+            plan.set_shape(c["Cb"], c["R"], c["erow"], c["ecol"])
+            return dict(c, cached=True)
+        except CmError:
+            pass  # a shape this build no longer offers (32-element transforms): tune again
     desc = plan.autotune(reps)
     cache[key] = desc
     if cache_path:

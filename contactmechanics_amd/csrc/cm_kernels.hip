@@ -2014,6 +2014,19 @@ bool has_generic(const LinePlan& lp) {
 // a generic radix run the 16-element GEN builds.
 int plan_emax(const LinePlan& lp) { return has_generic(lp) ? 16 : emax_bucket(lp.E); }
 
+// Solves with 32-element row transforms are disabled: a 2048 x 2048 FP64
+// solve at the explicit shape (Cb 2, R 2, 32-element rows, 8-element columns)
+// ended in a GPU memory fault whose cause was not found (the host build of the
+// same kernels runs that case with no out-of-bounds access).  Explicit
+// 32-element shapes are refused, the autotuner does not try them, and a solve
+// on a plan whose automatic shape needs 32-element rows (rows over 16384
+// points) returns CM_E_UNSUPPORTED.  32-element columns stay: the FP64
+// set-up and refresh passes of FP32 solves on 8192-point columns run them.
+bool uses_e32(const FftExec& ex) { return plan_emax(ex.row) == 32; }
+const char* const kE32Refused =
+    "32-element row transforms are disabled for solves (rows over 16384 points, or explicit 32-element launch "
+    "shapes): a solve with them ended in a GPU memory fault whose cause was not found";
+
 template <typename T>
 bool choose_shape(FftExec& ex, const Shape& sh) {
   Geom& g = ex.g;
@@ -2461,8 +2474,8 @@ int autotune(cm_plan* P, int reps, std::string& desc) {
   int tried = 0;
   for (int Cb : {1, 2, 4, 8})
     for (int R : {1, 2, 4, 8})
-      for (int er : {8, 16, 32})
-        for (int ec : {8, 16, 32}) {
+      for (int er : {8, 16})  // (32-element shapes are disabled for solves)
+        for (int ec : {8, 16}) {
           FftExec trial = P->ex;
           if (!choose_shape<T>(trial, Shape{Cb, R, er, ec})) continue;
           evaluate_ex<T>(P, trial, a, b, 0, st);  // warm-up (first launch of a kernel)
@@ -2740,6 +2753,7 @@ int session_begin(cm_plan* P, int B, const cm_ccg_params* prm, const double* con
                   const uint8_t* const* surfs, void* const* us, void* const* fs, uint8_t* const* cms,
                   void* stream, cm_session** out) {
   cm_ctx* ctx = P->ctx;
+  if (uses_e32(P->ex) || (P->prec != CM_FP64 && uses_e32(P->ex64))) return fail(ctx, CM_E_UNSUPPORTED, kE32Refused);
   for (int b = 1; b < B; ++b)
     if (prm[b].init_mode != prm[0].init_mode)
       return fail(ctx, CM_E_ARG, "all instances of a batch must share init_mode");
@@ -3066,6 +3080,7 @@ int solve_noalloc(cm_plan* P, const cm_ccg_params* prm, const double* h, const u
                   uint8_t* cm, cm_ccg_stats* stats, void* stream) {
   cm_ctx* ctx = P->ctx;
   if (!ctx->scratch) return fail(ctx, CM_E_HIP, "static scratch has no device address");
+  if (uses_e32(P->ex) || (P->prec != CM_FP64 && uses_e32(P->ex64))) return fail(ctx, CM_E_UNSUPPORTED, kE32Refused);
   cm_ccg_params q = *prm;
   q.use_graph = 0;
   cm_session S;
@@ -3240,6 +3255,7 @@ int refuse_fp32_shapes(cm_plan* plan) {
   return fail(plan->ctx, CM_E_UNSUPPORTED, "FP32 plans use the automatic launch shape");
 }
 
+
 int cm_plan_autotune(cm_plan* plan, int reps, char* out, size_t out_len) {
   if (!plan || reps < 1) return fail(plan ? plan->ctx : nullptr, CM_E_ARG, "bad argument");
   if (plan->prec != CM_FP64) return refuse_fp32_shapes(plan);
@@ -3256,6 +3272,7 @@ int cm_plan_autotune(cm_plan* plan, int reps, char* out, size_t out_len) {
 int cm_plan_set_shape(cm_plan* plan, int Cb, int R, int erow, int ecol) {
   if (!plan) return fail(nullptr, CM_E_ARG, "null plan");
   if (plan->prec != CM_FP64) return refuse_fp32_shapes(plan);
+  if (erow == 32 || ecol == 32) return fail(plan->ctx, CM_E_UNSUPPORTED, kE32Refused);
   FftExec trial = plan->ex;
   const bool ok = plan->prec == CM_FP64 ? choose_shape<double>(trial, Shape{Cb, R, erow, ecol})
                                         : choose_shape<float>(trial, Shape{Cb, R, erow, ecol});

@@ -74,3 +74,35 @@ def test_fp32_plans_keep_automatic_shape(gpu):
         g.plan.set_shape(1, 1, 16, 16)
     d = tune(g, cache_path=None)
     assert not d["tuned"] and d["Cb"] == g.plan.shape()["Cb"]
+
+
+def test_32_element_shapes_are_refused(gpu):
+    # a solve at a 32-element launch shape faulted the GPU (see the skipped
+    # case below): explicit 32-element shapes are refused and the tuner does
+    # not try them; the automatic shape stays usable
+    g = PeriodicFFTElasticHalfSpace((256, 256), 1.0, (1.0, 1.0))
+    auto = g.plan.shape()
+    for shp in [(2, 2, 32, 8), (2, 2, 8, 32), (1, 1, 32, 32)]:
+        with pytest.raises(_native.CmError):
+            g.plan.set_shape(*shp)
+    assert g.plan.shape() == auto
+    d = tune(g, cache_path=None, reps=2)
+    assert d["erow"] != 32 and d["ecol"] != 32
+
+
+# A 2048^2 FP64 solve at the explicit shape (Cb 2, R 2, 32-element rows,
+# 8-element columns) ended in a GPU memory fault; the host build of the same
+# kernels ran that case (set-up and one iteration, and every shape at 256^2)
+# with no out-of-bounds access, so the cause is not found.  Kept, skipped,
+# until it is shown and fixed; the library refuses such solves meanwhile.
+@pytest.mark.skip(reason="a 2048^2 FP64 solve with 32-element row transforms faulted the GPU; cause not found, "
+                         "the library refuses 32-element shapes for solves")
+def test_solve_with_32_element_rows(gpu):
+    from contactmechanics_amd import constrained_conjugate_gradients as ccg
+    from contactmechanics_amd import workloads
+    cfg = workloads.config2(2048)
+    g = PeriodicFFTElasticHalfSpace((2048, 2048), 1.0, (1.0, 1.0))
+    g.plan.set_shape(2, 2, 32, 8)
+    r = ccg(g, cfg["heights"], external_force=cfg["external_force"], pentol=0.0, forcetol=0.0, thermotol=0.0,
+            maxiter=20)
+    assert r.nit == 20
