@@ -1189,13 +1189,24 @@ __global__ void __launch_bounds__(1024) k_bisect(Geom g, InstArr<T> ia) {
   __shared__ double s_cnt;
   const size_t Mt = (size_t)g.nx * g.ny;
   const double H = s->H, F = s->F;
+  // (row, column) of this thread's first point; later points step by the
+  // block size with a carry into the row instead of a division per point
+  const int i0 = (int)(threadIdx.x / (unsigned)g.ny), j0 = (int)threadIdx.x - i0 * g.ny;
+  const int di = (int)(blockDim.x / (unsigned)g.ny), dj = (int)blockDim.x - di * g.ny;
   auto fval = [&](double x) -> double {
     double v = 0.0;
+    int i = i0, j = j0;
     for (size_t q = threadIdx.x; q < Mt; q += blockDim.x) {
-      const int i = (int)(q / g.ny), j = (int)(q - (size_t)i * g.ny);
-      if (in.surf && !in.surf[q]) continue;
-      const double g0 = in.w[(size_t)i * g.My + j];  // the gap u - h
-      v += (g0 - x < 0.0) ? 1.0 : 0.0;
+      if (!in.surf || in.surf[q]) {
+        const double g0 = in.w[(size_t)i * g.My + j];  // the gap u - h
+        v += (g0 - x < 0.0) ? 1.0 : 0.0;
+      }
+      i += di;
+      j += dj;
+      if (j >= g.ny) {
+        j -= g.ny;
+        ++i;
+      }
     }
     int ops[1] = {RSUM};
     double unused = 0.0;
