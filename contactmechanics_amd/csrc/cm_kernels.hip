@@ -1312,14 +1312,17 @@ __device__ void tau_finalize(Scal* s, double sum_rt) {
 // Per-pixel force after the PK or mixing update, before projection
 // (:296 and :322-323).  Shared by phase B and the second transform.
 // f + (tau c) t multiplies by the mask as the reference does (a NaN in t
-// reaches f even where c = 0).
+// reaches f even where c = 0).  HARD = false: no instance of the launch has a
+// hardness, so every step is a PK step (mixing needs a hardness, :260) and
+// the hardness branches compile away.
+template <bool HARD>
 __device__ __forceinline__ double force_update(const Scal* s, double fv, double tv, bool c) {
-  if (s->mode == MODE_PK) return fv + (c ? s->tau : 0.0) * tv;
+  if (!HARD || s->mode == MODE_PK) return fv + (c ? s->tau : 0.0) * tv;
   return (1.0 - s->mix) * fv - (s->mix * s->H) * (c ? 1.0 : 0.0);
 }
 
-// Phase B: projection statistics (:332-389).
-template <typename T>
+// Phase B: projection statistics (:332-389).  HARD as in force_update.
+template <typename T, bool HARD = true>
 struct ElemPhaseB {
   using F = T;
   static constexpr int NS = 7;  // n_tensile, max_tensile, pad_max, A_fl, min_fl, sum_f2, nc
@@ -1348,16 +1351,16 @@ struct ElemPhaseB {
     double gap = 0.0;
     if (comp) gap = wv - s->offset;
     bool c = (cmv & kNext) != 0;
-    if (s->mode == MODE_MIX && comp) c = gap < 0.0;  // c[comp] = g < 0 (:322)
-    const double f1 = force_update(s, fv, tv, c);
+    if (HARD && s->mode == MODE_MIX && comp) c = gap < 0.0;  // c[comp] = g < 0 (:322)
+    const double f1 = force_update<HARD>(s, fv, tv, c);
     const bool tensile = f1 >= 0.0;
-    const bool flowing = s->has_H && f1 <= -s->H;
+    const bool flowing = HARD && s->has_H && f1 <= -s->H;
     if (tensile) {
       acc[0] += 1.0;
       acc[1] = red_apply(RMAX, acc[1], f1);
     }
     if (!comp) acc[2] = red_apply(RMAX, acc[2], fabs(f1));
-    if (s->H_truthy && flowing) {
+    if (HARD && s->H_truthy && flowing) {
       acc[3] += 1.0;
       acc[4] = red_apply(RMIN, acc[4], f1 + s->H);
     }
@@ -1372,7 +1375,7 @@ struct ElemPhaseB {
     const Scal* s = &sv;
     const uint8_t c0 = one(s, r.cp.c0, r.cm.a(), r.f.a, (double)r.t.a, r.w.a, acc);
     const uint8_t c1 = one(s, r.cp.c1, r.cm.b(), r.f.b, (double)r.t.b, r.w.b, acc);
-    if (s->mode == MODE_MIX) stm(in.cm + (size_t)row * g.My + col, c0, c1);
+    if (HARD && s->mode == MODE_MIX) stm(in.cm + (size_t)row * g.My + col, c0, c1);
   }
   __device__ void finalize(double* a, Scal* s) const {
     s->pad_pres = s->n_pad > 0 ? a[2] : 0.0;             // :344-346
@@ -1411,7 +1414,7 @@ struct ElemPhaseB {
 // The transform (precision TT) takes the new force (INC = false) or, for the
 // incremental displacement update of FP32 plans, the change of the force
 // f_new - f_old (INC = true), formed in FP64.
-template <typename T, typename TT, bool INC>
+template <typename T, typename TT, bool INC, bool HARD = true>
 struct ProdForce {
   using F = T;
   static constexpr int kBatch = kPairs;
@@ -1438,9 +1441,9 @@ struct ProdForce {
     double gap = 0.0;
     if (comp) gap = wv - s->offset;
     const bool c = (cmv & kNext) != 0;
-    const double f1 = force_update(s, fv, tv, c);
+    const double f1 = force_update<HARD>(s, fv, tv, c);
     const bool tensile = f1 >= 0.0;
-    const bool flowing = s->has_H && f1 <= -s->H;
+    const bool flowing = HARD && s->has_H && f1 <= -s->H;
     double fo = tensile ? 0.0 : f1;                       // :359
     if (s->load_ctl) {
       if (s->uniform)
@@ -1455,7 +1458,7 @@ struct ProdForce {
       // contact set of the next iteration from the final force (:239-249),
       // bits 2 and 3 for the displacement pass and the next iteration
       const bool cn0 = fo < 0.0;
-      const bool cn = cn0 && (!s->has_H || fo > -s->H);
+      const bool cn = cn0 && (!HARD || !s->has_H || fo > -s->H);
       cmv = (uint8_t)((c ? kCur : 0) | (gap < 0.0 ? kGneg : 0) | (cn ? kNext : 0) | (cn0 ? kNeg : 0));
       kadd(acc[0], acc[NS], fo);
     }
@@ -2681,6 +2684,7 @@ struct cm_session {
   size_t ws_bytes = 0;
   WsLayout L{};
   bool any_bisect = false;
+  bool any_hard = false;  // an instance has a hardness (else the PK-only passes run)
   int use_graph = 0;
   long long issued = 0;
   int maxiter = 0;
@@ -2704,18 +2708,18 @@ bool refresh_iteration(const cm_session* S, long long k) {
 // The second convolution of an iteration and the displacement pass: FP64 on
 // FP64 plans; FP32 plans take the FP64 transforms of the new forces on
 // refresh iterations and the FP32 transforms of the force change otherwise.
-template <typename T>
+template <typename T, bool HARD>
 void enqueue_displacement_update(cm_session* S, InstArr<T> ia, bool refresh) {
   const int B = S->B;
   hipStream_t st = S->st;
   if (refresh) {
     const FftExec& e64 = S->ex64;
-    launch_r2c<double, ProdForce<T, double, false>>(e64, ia, B, 6, st);
+    launch_r2c<double, ProdForce<T, double, false, HARD>>(e64, ia, B, 6, st);
     launch_col<double, 0, T>(e64, ia, B, 0, 1, st);
     launch_c2r<double, ConsDisp<T, double, false>>(e64, ia, B, 7, st);
   } else if constexpr (!std::is_same<T, double>::value) {
     const FftExec& ex = S->ex;
-    launch_r2c<T, ProdForce<T, T, true>>(ex, ia, B, 6, st);
+    launch_r2c<T, ProdForce<T, T, true, HARD>>(ex, ia, B, 6, st);
     launch_col<T, 0>(ex, ia, B, 0, 1, st);
     launch_c2r<T, ConsDisp<T, T, true>>(ex, ia, B, 7, st);
   }
@@ -2733,8 +2737,13 @@ void enqueue_iteration(cm_session* S, bool refresh) {
   if (S->any_bisect) hipLaunchKernelGGL((k_bisect<T>), dim3(B), dim3(1024), 0, st, ex.g, ia);
   launch_r2c<T, ProdDir<T>>(ex, ia, B, 3, st);
   launch_col<T, 3>(ex, ia, B, 0, 2, st);  // tau from the spectrum: no inverse pass
-  launch_elem<T, ElemPhaseB<T>>(ex, nbe, ia, B, 5, st);
-  enqueue_displacement_update<T>(S, ia, refresh);
+  if (S->any_hard) {
+    launch_elem<T, ElemPhaseB<T, true>>(ex, nbe, ia, B, 5, st);
+    enqueue_displacement_update<T, true>(S, ia, refresh);
+  } else {
+    launch_elem<T, ElemPhaseB<T, false>>(ex, nbe, ia, B, 5, st);
+    enqueue_displacement_update<T, false>(S, ia, refresh);
+  }
 }
 
 void session_free(cm_session* S);
@@ -2864,6 +2873,7 @@ int session_init(cm_session* S, const cm_ccg_params* prm, const double* const* h
       s.status = CM_MAXITER;
     }
     S->any_bisect = S->any_bisect || (q.has_hardness && q.load_control);
+    S->any_hard = S->any_hard || q.has_hardness;
     S->maxiter = std::max(S->maxiter, q.maxiter);
   }
   hipStream_t st = S->st;
@@ -2977,18 +2987,27 @@ int session_profile(cm_session* S, int n, float* us) {
     HIPCHK(ctx, hipEventRecord(ev[k++], st));
     // (tau comes from the spectrum in the column pass: no c2r pass)
     HIPCHK(ctx, hipEventRecord(ev[k++], st));
-    launch_elem<T, ElemPhaseB<T>>(ex, nbe, ia, B, 5, st);
+    if (S->any_hard)
+      launch_elem<T, ElemPhaseB<T, true>>(ex, nbe, ia, B, 5, st);
+    else
+      launch_elem<T, ElemPhaseB<T, false>>(ex, nbe, ia, B, 5, st);
     // the displacement update as between refreshes: FP64 transforms of the
     // new forces (FP64 plans), FP32 transforms of the force change (FP32)
     HIPCHK(ctx, hipEventRecord(ev[k++], st));
     if constexpr (std::is_same<T, double>::value) {
-      launch_r2c<T, ProdForce<T, T, false>>(ex, ia, B, 6, st);
+      if (S->any_hard)
+        launch_r2c<T, ProdForce<T, T, false, true>>(ex, ia, B, 6, st);
+      else
+        launch_r2c<T, ProdForce<T, T, false, false>>(ex, ia, B, 6, st);
       HIPCHK(ctx, hipEventRecord(ev[k++], st));
       launch_col<T, 0>(ex, ia, B, 0, 1, st);
       HIPCHK(ctx, hipEventRecord(ev[k++], st));
       launch_c2r<T, ConsDisp<T, T, false>>(ex, ia, B, 7, st);
     } else {
-      launch_r2c<T, ProdForce<T, T, true>>(ex, ia, B, 6, st);
+      if (S->any_hard)
+        launch_r2c<T, ProdForce<T, T, true, true>>(ex, ia, B, 6, st);
+      else
+        launch_r2c<T, ProdForce<T, T, true, false>>(ex, ia, B, 6, st);
       HIPCHK(ctx, hipEventRecord(ev[k++], st));
       launch_col<T, 0>(ex, ia, B, 0, 1, st);
       HIPCHK(ctx, hipEventRecord(ev[k++], st));
