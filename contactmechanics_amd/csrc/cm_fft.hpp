@@ -550,6 +550,20 @@ __device__ __forceinline__ double red_apply(int op, double a, double b) {
   return op == RMAX ? (a > b ? a : b) : (a < b ? a : b);
 }
 
+// Running max (min) over a thread's points with red_apply's NaN propagation
+// at a third of its instructions: the hardware max (which returns the other
+// operand when one is NaN) plus the last NaN seen, kept in nan (a max or min
+// slot's otherwise unused correction term) and folded into the value once per
+// thread (fold_nan) before the cross-thread reduction.
+__device__ __forceinline__ void kmax(double& a, double& nan, double x) {
+  a = __builtin_fmax(a, x);
+  nan = (x != x) ? x : nan;
+}
+__device__ __forceinline__ void kmin(double& a, double& nan, double x) {
+  a = __builtin_fmin(a, x);
+  nan = (x != x) ? x : nan;
+}
+
 // s + x with the rounding error of the addition collected in c (Neumaier).
 __device__ __forceinline__ void kadd(double& s, double& c, double x) {
   const double t = s + x;

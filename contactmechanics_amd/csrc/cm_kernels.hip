@@ -144,6 +144,21 @@ constexpr int kCtrSlot = kCtrLine * (kTicketGroups + 1); // counters of one redu
 // FP32 plans: default number of iterations per FP64 displacement refresh.
 constexpr int kRefreshEvery = 8;
 
+// A NaN that a thread's kmax/kmin remembered in a max or min slot's
+// correction term becomes the slot's value (red_apply's result), and the
+// term goes back to zero (the cross-thread reduction carries no correction
+// for such slots).
+template <class P>
+__device__ __forceinline__ void fold_nan(double* acc) {
+  constexpr int NS = P::NS;
+#pragma unroll
+  for (int i = 0; i < NS; ++i)
+    if (P::op(i) == RMAX || P::op(i) == RMIN) {
+      if (acc[NS + i] != acc[NS + i]) acc[i] = acc[NS + i];
+      acc[NS + i] = 0.0;
+    }
+}
+
 // Cross-block reduction: every block reduces NS slots, writes its partials,
 // takes a ticket; the block drawing the last ticket reduces all partials in a
 // fixed order (deterministic result) and returns true.
@@ -562,6 +577,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_r2
     }
   }
   if constexpr (P::NS > 0) {
+    fold_nan<P>(acc);
     if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blockIdx.x))
       finalize_block(p, acc, in.sc);
   }
@@ -880,6 +896,7 @@ __global__ void __launch_bounds__(fft_block_max(EMAX), fft_min_waves(EMAX)) k_c2
     }
   }
   if constexpr (P::NS > 0) {
+    fold_nan<P>(acc);
     if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blk))
       finalize_block(p, acc, in.sc);
   }
@@ -924,6 +941,7 @@ __global__ void __launch_bounds__(256, 4) k_elem(Geom g, InstArr<T> ia, int ctr_
     }
   }
   if constexpr (P::NS > 0) {
+    fold_nan<P>(acc);
     if (reduce_blocks<P>(acc, in.part, in.ctr + ctr_slot * kCtrSlot, gridDim.x, blockIdx.x))
       finalize_block(p, acc, in.sc);
   }
@@ -1357,12 +1375,12 @@ struct ElemPhaseB {
     const bool flowing = HARD && s->has_H && f1 <= -s->H;
     if (tensile) {
       acc[0] += 1.0;
-      acc[1] = red_apply(RMAX, acc[1], f1);
+      kmax(acc[1], acc[NS + 1], f1);
     }
-    if (!comp) acc[2] = red_apply(RMAX, acc[2], fabs(f1));
+    if (!comp) kmax(acc[2], acc[NS + 2], fabs(f1));
     if (HARD && s->H_truthy && flowing) {
       acc[3] += 1.0;
-      acc[4] = red_apply(RMIN, acc[4], f1 + s->H);
+      kmin(acc[4], acc[NS + 4], f1 + s->H);
     }
     if (comp) {
       kadd(acc[5], acc[NS + 5], tensile ? 0.0 : f1);
@@ -1546,10 +1564,10 @@ struct ConsDisp {
       wn = wo + (double)v;
     else
       wn = comp ? (double)v - hv : (double)v;
-    acc[0] = red_apply(RMAX, acc[0], fabs(wn - wo));
+    kmax(acc[0], acc[NS + 0], fabs(wn - wo));
     if (comp) {
       const double pen = (cmv & kCur) ? (cx.offset - wn) : 0.0;
-      acc[1] = red_apply(RMAX, acc[1], pen);
+      kmax(acc[1], acc[NS + 1], pen);
       // contact counts and shifted gap sums of the next iteration, shift =
       // this offset (:239-273)
       const bool cn = (cmv & kNext) != 0;
@@ -1558,8 +1576,8 @@ struct ConsDisp {
       const double d = wn - cx.offset;
       if (cn) acc[4] += d;                  // sum over the contact set only (:269)
       acc[5] += (cn ? 1.0 : 0.0) * d * d;   // c * g * g: a NaN gap poisons G as in :273
-      acc[6] = red_apply(RMAX, acc[6], wn);
-      acc[7] = red_apply(RMIN, acc[7], wn);
+      kmax(acc[6], acc[NS + 6], wn);
+      kmin(acc[7], acc[NS + 7], wn);
     }
   }
   __device__ void take(const Geom& g, int row, int col, const Rec& r, TT a, TT b, double* acc, const Ctx& cx) const {
