@@ -13,8 +13,10 @@ Extra keywords (not in the reference):
 ``return_device``
     keep ``x``/``jac``/``active_set`` as torch tensors on the device.
 ``iters_per_sync``
-    host synchronisation interval; 0 = adaptive.  A ``callback`` or an active
-    ``logger`` forces 1 (the host then sees every iteration).
+    host synchronisation interval; 0 = adaptive.  A ``callback`` forces 1 (it
+    receives every iteration's forces).  An active ``logger`` alone keeps the
+    adaptive chunks: the device keeps the statistics of each chunk's
+    iterations (a ring of 64 records) and every iteration is logged from them.
 ``use_graph``
     replay chunks of iterations from a captured hipGraph.
 ``refresh_every``
@@ -195,7 +197,11 @@ def constrained_conjugate_gradients(substrate, topography, hardness=None, extern
     u, f = gm.fwd(u), gm.fwd(f)
     cmask = torch.zeros(gm.plan_shape, dtype=torch.uint8, device=dev)
 
-    per_iteration = callback is not None or getattr(logger, "active", False)
+    # a callback takes each iteration's forces, so the host checks in after
+    # every iteration; a logger alone takes statistics, which the device keeps
+    # for the last iterations of a chunk (its log ring), so chunks stay long
+    needs_fields = callback is not None
+    per_iteration = needs_fields or getattr(logger, "active", False)
     params = _native.CcgParams(
         load_control=1 if external_force is not None else 0,
         external_force=float(external_force) * tile if external_force is not None else 0.0,
@@ -205,8 +211,8 @@ def constrained_conjugate_gradients(substrate, topography, hardness=None, extern
         pentol=float(pentol), forcetol=float(forcetol), thermotol=float(thermotol),
         mixfac=float(mixfac), mixdecfac=float(mixdecfac), minmixsteps=int(minmixsteps),
         maxiter=int(maxiter), init_mode=init_mode,
-        iters_per_sync=1 if per_iteration else int(iters_per_sync),
-        use_graph=0 if per_iteration else int(bool(use_graph)), refresh_every=int(refresh_every))
+        iters_per_sync=1 if needs_fields else int(iters_per_sync),
+        use_graph=0 if needs_fields else int(bool(use_graph)), refresh_every=int(refresh_every))
     stats = _native.CcgStats()
 
     n_surf = int(valid.sum()) if valid is not None else nb_surface_pts

@@ -141,6 +141,11 @@ constexpr int kTicketGroups = 16;
 constexpr int kCtrLine = 16;                             // unsigned per 64-byte line
 constexpr int kCtrSlot = kCtrLine * (kTicketGroups + 1); // counters of one reduction slot
 
+// Records of the last kLogRing iterations kept per instance behind its scalar
+// record (the log ring): a chunk of up to kLogRing iterations runs with no
+// host check in between, and the host still reports every iteration.
+constexpr int kLogRing = 64;
+
 // FP32 plans: default number of iterations per FP64 displacement refresh.
 constexpr int kRefreshEvery = 8;
 
@@ -435,8 +440,11 @@ __device__ __forceinline__ int xcd_row_block(int b, int n) {
 // cost LDS latency instead of one memory round trip each.  Called by every
 // thread of the last block; when it runs, every other block of the kernel has
 // read the record already (each reads it when it starts, before its ticket)
-// and no kernel behind this one has started.
-template <typename F>
+// and no kernel behind this one has started.  With RING, the record as the
+// logic left it is also stored in the instance's log ring (kLogRing records
+// behind the scalar record, slot it % kLogRing), so the host can report every
+// iteration of a chunk.
+template <bool RING = false, typename F>
 __device__ __forceinline__ void with_scal_in_lds(Scal* gs, F&& logic) {
   __shared__ Scal ls;
   constexpr int nw = (int)(sizeof(Scal) / sizeof(unsigned));
@@ -449,6 +457,11 @@ __device__ __forceinline__ void with_scal_in_lds(Scal* gs, F&& logic) {
   __syncthreads();
   const auto out = (__attribute__((address_space(1))) unsigned*)gs;
   for (int i = threadIdx.x; i < nw; i += blockDim.x) out[i] = dst[i];
+  if constexpr (RING) {
+    const int slot = (ls.it % kLogRing + kLogRing) % kLogRing;
+    const auto rg = (__attribute__((address_space(1))) unsigned*)(gs + 1 + slot);
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) rg[i] = dst[i];
+  }
 }
 // Functors whose finalize runs the long scalar logic (P::kScalInLds) use the
 // LDS copy; the others write a field or two directly.
@@ -456,6 +469,11 @@ template <class P, class = void>
 struct scal_in_lds : std::false_type {};
 template <class P>
 struct scal_in_lds<P, std::void_t<decltype(P::kScalInLds)>> : std::bool_constant<P::kScalInLds> {};
+// Functors that end an iteration (P::kLogsRing) also store the record in the log ring.
+template <class P, class = void>
+struct logs_ring : std::false_type {};
+template <class P>
+struct logs_ring<P, std::void_t<decltype(P::kLogsRing)>> : std::bool_constant<P::kLogsRing> {};
 
 // The last block's finalize on the reduced slots (a copy, so the accumulator
 // array itself never has its address taken and stays in registers: the
@@ -467,7 +485,7 @@ __device__ __forceinline__ void finalize_block(const P& p, const double* acc, Sc
 #pragma unroll
   for (int i = 0; i < P::NS; ++i) fin[i] = P::op(i) == RCSUM ? kvalue(acc[i], acc[P::NS + i]) : acc[i];
   if constexpr (scal_in_lds<P>::value)
-    with_scal_in_lds(gs, [&](Scal* s) { p.finalize(fin, s); });
+    with_scal_in_lds<logs_ring<P>::value>(gs, [&](Scal* s) { p.finalize(fin, s); });
   else if (threadIdx.x == 0)
     p.finalize(fin, gs);
 }
@@ -1525,6 +1543,7 @@ struct ConsDisp {
   // four waves per SIMD), four with FP32 transforms (88 VGPRs with two)
   static constexpr int kBatch = sizeof(TT) == 4 ? 4 : 2;
   static constexpr bool kScalInLds = true;  // convergence tests and the next iteration's start
+  static constexpr bool kLogsRing = true;   // the end of an iteration
   __device__ static constexpr int op(int i) { return i < 2 ? RMAX : phase_a_op(i - 2); }
   struct Rec {
     P2<double> w;
@@ -2642,7 +2661,7 @@ WsLayout ws_layout(const cm_plan* P, int B) {
   WsLayout L{};
   L.st_t = al256(M * w);
   L.st_S = al256((M / 2) * 2 * w);
-  L.st_sc = al256(sizeof(Scal));
+  L.st_sc = al256(sizeof(Scal) * (1 + kLogRing));  // the record and its log ring
   L.st_part = al256(nparts * sizeof(double));
   L.st_ctr = al256(8 * kCtrSlot * sizeof(unsigned));
   // FP32 plans: FP64 gap and forces, FP64 spectrum of the refresh transforms
@@ -3061,6 +3080,15 @@ int session_poll(cm_session* S, cm_ccg_stats* stats, int* all_done) {
   return 0;
 }
 
+// The log ring of instance b (after a poll: the stream is idle).
+int session_ring(cm_session* S, int b, std::vector<Scal>& out) {
+  cm_ctx* ctx = S->P->ctx;
+  out.resize(kLogRing);
+  const char* base = (const char*)S->ws + S->L.sc + (size_t)b * S->L.st_sc + sizeof(Scal);
+  HIPCHK(ctx, hipMemcpy(out.data(), base, sizeof(Scal) * kLogRing, hipMemcpyDeviceToHost));
+  return 0;
+}
+
 void session_free(cm_session* S) {
   if (!S) return;
   cm_ctx* ctx = S->P->ctx;
@@ -3085,7 +3113,12 @@ int solve_driver(cm_plan* P, int B, const cm_ccg_params* prm, const double* cons
   if (rc) return rc;
   const int ips = prm[0].iters_per_sync;
   int chunk = ips > 0 ? ips : 4;
+  // adaptive chunks with a progress function: every iteration of a chunk is
+  // reported from the log ring (the chunks never exceed the ring)
+  const bool from_ring = progress && ips <= 0;
   std::vector<cm_ccg_stats> tmp(B);
+  std::vector<Scal> ring;
+  long long reported = 0;
   int result = 0;
   while (true) {
     rc = P->prec == CM_FP64 ? session_step<double>(S, chunk) : session_step<float>(S, chunk);
@@ -3098,13 +3131,31 @@ int solve_driver(cm_plan* P, int B, const cm_ccg_params* prm, const double* cons
       result = rc;
       break;
     }
-    if (progress) progress(user, &tmp[0]);
+    if (from_ring) {
+      if ((rc = session_ring(S, 0, ring))) {
+        result = rc;
+        break;
+      }
+      for (long long it = reported + 1; it <= tmp[0].nit; ++it) {
+        const Scal& r = ring[it % kLogRing];
+        if (it == tmp[0].nit) {
+          progress(user, &tmp[0]);  // the latest record (also an iteration ended outside the ring)
+        } else if (r.it == it) {
+          cm_ccg_stats st;
+          fill_stats(r, &st);
+          progress(user, &st);
+        }
+      }
+      reported = tmp[0].nit;
+    } else if (progress) {
+      progress(user, &tmp[0]);
+    }
     if (done) break;
     if (S->issued > (long long)S->maxiter + 8) {
       result = fail(P->ctx, CM_E_ARG, "solver did not terminate within maxiter");
       break;
     }
-    if (ips <= 0) chunk = std::min(chunk * 2, 64);
+    if (ips <= 0) chunk = std::min(chunk * 2, kLogRing);
   }
   if (result == 0) {
     P->prec == CM_FP64 ? session_finish<double>(S) : session_finish<float>(S);
@@ -3169,7 +3220,7 @@ int solve_noalloc(cm_plan* P, const cm_ccg_params* prm, const double* h, const u
       result = fail(ctx, CM_E_ARG, "solver did not terminate within maxiter");
       break;
     }
-    if (ips <= 0) chunk = std::min(chunk * 2, 64);
+    if (ips <= 0) chunk = std::min(chunk * 2, kLogRing);
   }
   if (result == 0) {
     session_finish<T>(&S);

@@ -110,7 +110,10 @@ typedef struct cm_ccg_stats {
   double nc_count;
 } cm_ccg_stats;
 
-/* Called by cm_ccg_solve after every host synchronisation (see iters_per_sync). */
+/* Called by cm_ccg_solve with the statistics of instance 0: with
+ * iters_per_sync > 0, once per host synchronisation (the latest iteration);
+ * with iters_per_sync == 0 (adaptive chunks), once for every iteration of
+ * each chunk, from the device's log ring of the last 64 iterations. */
 typedef void (*cm_progress_fn)(void* user, const cm_ccg_stats* stats);
 
 const char* cm_last_error(cm_ctx* ctx);
