@@ -177,6 +177,53 @@ def oracle_jac(prob):
     return r.jac
 
 
+def bench_sweep(args):
+    """C3: warm-started plastic load sweep on the free 1024^2 topography (2048^2
+    computational grid), FP64, each load from the previous solution with the
+    plastic displacement carried over."""
+    import numpy as np
+    import torch
+    from contactmechanics_amd import (FreeFFTElasticHalfSpace, PlasticTopography, Topography, make_plastic_system,
+                                      workloads)
+    cfg = workloads.config3()
+    n = cfg["n"]
+    h = cfg["heights"]
+    system = make_plastic_system(FreeFFTElasticHalfSpace((n, n), 1.0, (1.0, 1.0)),
+                                 PlasticTopography(Topography(h, (1.0, 1.0)), cfg["hardness_pressure"]))
+    u0 = np.zeros((2 * n, 2 * n))
+    u0[:n, :n] = np.maximum(h - h.max() + 1e-4, 0.0)  # contact at the highest point only
+    # warm-up: the first load's first iterations (one-time module and allocation costs)
+    system.minimize_proxy(external_force=cfg["loads"][0], initial_displacements=u0, pentol=cfg["pentol"],
+                          forcetol=cfg["forcetol"], thermotol=1e-10, maxiter=3)
+    system.surface.plastic_displ[...] = 0.0
+    torch.cuda.synchronize()
+    disp, loads, seconds = u0, [], 0.0
+    for F in cfg["loads"]:
+        # timed: the call a sweep makes per load (host arrays in and out, the
+        # plastic update of the topography); the property checks below are not
+        t0 = time.perf_counter()
+        res = system.minimize_proxy(external_force=F, initial_displacements=disp, pentol=cfg["pentol"],
+                                    forcetol=cfg["forcetol"], thermotol=1e-10, maxiter=20000)
+        seconds += time.perf_counter() - t0
+        disp = system.disp
+        p = system.force / system.area_per_pt
+        loads.append(dict(load=float(F), nit=int(res.nit), success=bool(res.success),
+                          area=float(np.mean(p > 0)), plastic_px=int(np.count_nonzero(system.surface.plastic_displ)),
+                          max_p_over_H=float(np.max(p) / cfg["hardness_pressure"]),
+                          force_rel_err=float(abs(np.sum(system.force) - F) / F)))
+        print(f"# c3 load {F:.3e}: {loads[-1]}", file=sys.stderr, flush=True)
+    nit = sum(x["nit"] for x in loads)
+    M = (2 * n) ** 2
+    line = {
+        "metric": "C3 plastic load sweep, grid-pt-iters/s (FP64, warm-started)",
+        "value": M * nit / seconds, "unit": "grid-pt-iters/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": cfg["name"]},
+        "seconds": seconds, "iterations": nit, "loads": loads,
+    }
+    print(json.dumps(line))
+
+
 def bench_batch(args):
     """C4: 256 independent instances of uneven size in one batched run."""
     import numpy as np
@@ -275,8 +322,9 @@ def bench_serve(args):
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5"],
-                    help="c2 (default): single 2048^2 solve; c4: batched uneven solves; c5: served stream")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+                    help="c2 (default): single 2048^2 solve; c3: plastic load sweep; c4: batched uneven solves; "
+                         "c5: served stream")
     ap.add_argument("--count", type=int, default=256, help="c4 instances / c5 requests")
     ap.add_argument("--rate", type=float, default=2.0, help="c5 arrival rate (requests/s)")
     ap.add_argument("--side-max", type=int, default=4096, help="c5 largest grid side")
@@ -302,6 +350,8 @@ def main():
     from contactmechanics_amd import PeriodicFFTElasticHalfSpace, workloads
 
     torch.cuda.set_device(0)
+    if args.workload == "c3":
+        return bench_sweep(args)
     if args.workload == "c4":
         return bench_batch(args)
     if args.workload == "c5":

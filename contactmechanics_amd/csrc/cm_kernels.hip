@@ -63,6 +63,10 @@ struct Scal {
   int load_ctl, has_H, H_truthy, minmixsteps, maxiter;
   int it, nfev, delta, delta_str, mode, mixsteps, done, status;
   int has_last_tf, has_last_mh, use_beta, do_nc, uniform, pad_;
+  // batched bisection across launches: bracket end xa, step dm, f(xa), steps
+  // taken, state (1 running, 2 finished)
+  double bis_xa, bis_dm, bis_fa;
+  int bis_steps, bis_state;
 };
 
 // One instance of a solve (or of a plain operator evaluation).  T is the
@@ -164,6 +168,23 @@ __device__ __forceinline__ void fold_nan(double* acc) {
     }
 }
 
+// Ticket of block bid of nblocks, taken by one thread once the block's
+// partials are stored and complete: true in the block that takes the last
+// ticket (that block resets the top counter when it is done with the
+// partials; the group counters reset themselves).
+__device__ __forceinline__ bool take_ticket(unsigned* ctr, int nblocks, int bid) {
+  const int g = bid % kTicketGroups;
+  const int gsize = nblocks / kTicketGroups + (g < nblocks % kTicketGroups ? 1 : 0);
+  const int ngroups = nblocks < kTicketGroups ? nblocks : kTicketGroups;
+  unsigned* grp = ctr + kCtrLine * (1 + g);
+  bool last = false;
+  if (atomicAdd(grp, 1u) == (unsigned)(gsize - 1)) {  // this block completes its group
+    __hip_atomic_store(grp, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    last = atomicAdd(ctr, 1u) == (unsigned)(ngroups - 1);
+  }
+  return last;
+}
+
 // Cross-block reduction: every block reduces NS slots, writes its partials,
 // takes a ticket; the block drawing the last ticket reduces all partials in a
 // fixed order (deterministic result) and returns true.
@@ -197,16 +218,7 @@ __device__ __forceinline__ bool reduce_blocks(double* acc, double* part, unsigne
                            __HIP_MEMORY_SCOPE_AGENT);
     }
     wait_stores_done();
-    const int g = bid % kTicketGroups;
-    const int gsize = nblocks / kTicketGroups + (g < nblocks % kTicketGroups ? 1 : 0);
-    const int ngroups = nblocks < kTicketGroups ? nblocks : kTicketGroups;
-    unsigned* grp = ctr + kCtrLine * (1 + g);
-    int last = 0;
-    if (atomicAdd(grp, 1u) == (unsigned)(gsize - 1)) {  // this block completes its group
-      __hip_atomic_store(grp, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      last = atomicAdd(ctr, 1u) == (unsigned)(ngroups - 1);
-    }
-    s_last = last;
+    s_last = take_ticket(ctr, nblocks, bid);
   }
   __syncthreads();
   if (!s_last) return false;
@@ -1213,30 +1225,206 @@ struct ElemPhaseA {
   }
 };
 
-// scipy.optimize.bisect (xtol=2e-12, rtol=4*eps, maxiter=100) of
-// x -> count(g - x < 0) * H - F over the computational region (:316-318).
-// One block per instance.
+// scipy.optimize.bisect(f, max g, min g, xtol=2e-12, rtol=4*eps, maxiter=100)
+// of f(x) = count(g - x < 0) * H - F over the computational region
+// (:316-318), with its steps and stopping rule:
+//   dm = xb - xa; repeat: dm *= 0.5; xm = xa + dm; fm = f(xm);
+//   if fm * fa >= 0: xa = xm;  if fm == 0 or |dm| < xtol + rtol |xm|: xm.
+// f is evaluated as numpy does, fl(fl(cnt * H) - F), with no fused multiply
+// -add; count(g - x < 0) equals count(g < x) for every pair of doubles
+// (a difference of distinct finite doubles is never rounded to zero; NaN
+// compares false either way).
+//
+// Batched rounds: the points that the next kBisDepth steps can visit form a
+// binary tree fixed by (xa, dm) alone (each step halves dm and moves xa to xm
+// or not), so one pass over the gaps counts the gaps below all 2^depth - 1 of
+// them (in round 0 also below the brackets), and the block that finishes the
+// pass replays the steps with those counts: the same points, counts and
+// decisions as the serial bisection, in ceil(steps / kBisDepth) passes over
+// the gaps by the whole grid instead of one pass per step by one block.
+constexpr int kBisDepth = 8;                          // steps per round
+constexpr int kBisTree = (1 << kBisDepth) - 1;        // points of a round's tree
+constexpr int kBisPts = kBisTree + 2;                 // round 0: and the brackets
+constexpr int kBisStride = 264;                       // bins (kBisPts + 1) per block partial, padded
+constexpr int kBisRounds = 6;                         // batched steps: kBisRounds * kBisDepth
+constexpr int kBisBlockMax = 128;
+#ifndef CM_BIS_PER_BLOCK
+#define CM_BIS_PER_BLOCK 8192  // the CPU emulation builds with fewer, so small grids span several blocks
+#endif
+constexpr int kBisPerBlock = CM_BIS_PER_BLOCK;        // gaps per block (32 per thread)
+static_assert(kBisPts + 1 <= kBisStride && kBisPts <= 512, "bisection tables");
+
+int bisect_blocks(const Geom& g) {
+  const size_t Mt = (size_t)g.nx * g.ny;
+  return (int)std::max<size_t>(1, std::min<size_t>(kBisBlockMax, (Mt + kBisPerBlock - 1) / kBisPerBlock));
+}
+
+__device__ __forceinline__ double bis_f(double cnt, double H, double F) {
+#pragma clang fp contract(off)
+  return cnt * H - F;
+}
+__device__ __forceinline__ bool bis_stop(double fm, double dm, double xm) {
+#pragma clang fp contract(off)
+  return fm == 0.0 || fabs(dm) < 2e-12 + 8.881784197001252e-16 * fabs(xm);
+}
+
+// Point t (level order) of the tree of the next kBisDepth steps from (xa, dm):
+// the bits of its path are the decisions "xa moved to xm".
+__device__ __forceinline__ double bis_tree_point(double xa, double dm, int t) {
+  const int l = 31 - __clz(t + 1);
+  const int p = t + 1 - (1 << l);
+  double x = xa, d = dm;
+  for (int lev = 0; lev < l; ++lev) {
+    d *= 0.5;
+    const double xm = x + d;
+    if ((p >> (l - 1 - lev)) & 1) x = xm;
+  }
+  d *= 0.5;
+  return x + d;
+}
+
+// count(g < x) over the computational points by one whole block (every
+// thread gets the count).
 template <typename T>
-__global__ void __launch_bounds__(1024) k_bisect(Geom g, InstArr<T> ia) {
-  const Inst<T>& in = ia.get(blockIdx.x);
-  Scal* s = in.sc;
-  if (s->done || s->mode != MODE_MIX || !s->load_ctl) return;
-  __shared__ double scr[2 * 8 * 16];
+__device__ double bis_count_block(const Geom& g, const Inst<T>& in, double x, double* scr) {
   __shared__ double s_cnt;
   const size_t Mt = (size_t)g.nx * g.ny;
-  const double H = s->H, F = s->F;
-  // (row, column) of this thread's first point; later points step by the
-  // block size with a carry into the row instead of a division per point
   const int i0 = (int)(threadIdx.x / (unsigned)g.ny), j0 = (int)threadIdx.x - i0 * g.ny;
   const int di = (int)(blockDim.x / (unsigned)g.ny), dj = (int)blockDim.x - di * g.ny;
-  auto fval = [&](double x) -> double {
-    double v = 0.0;
-    int i = i0, j = j0;
-    for (size_t q = threadIdx.x; q < Mt; q += blockDim.x) {
-      if (!in.surf || in.surf[q]) {
-        const double g0 = in.w[(size_t)i * g.My + j];  // the gap u - h
-        v += (g0 - x < 0.0) ? 1.0 : 0.0;
+  double v = 0.0;
+  int i = i0, j = j0;
+  for (size_t q = threadIdx.x; q < Mt; q += blockDim.x) {
+    if (!in.surf || in.surf[q]) v += in.w[(size_t)i * g.My + j] < x ? 1.0 : 0.0;
+    i += di;
+    j += dj;
+    if (j >= g.ny) {
+      j -= g.ny;
+      ++i;
+    }
+  }
+  int ops[1] = {RSUM};
+  double unused = 0.0;
+  block_reduce<1>(&v, &unused, ops, scr);
+  if (threadIdx.x == 0) s_cnt = v;
+  __syncthreads();
+  const double cnt = s_cnt;
+  __syncthreads();
+  return cnt;
+}
+
+// Serial bisection steps i.. by one whole block from (xa, dm, fa), one count
+// per step; writes the offset or the failure.
+template <typename T>
+__device__ void bis_serial(const Geom& g, const Inst<T>& in, Scal* s, double xa, double dm, double fa, int i,
+                           double* scr) {
+  const double H = s->H, F = s->F;
+  double res = 0.0;
+  bool ok = false;
+  for (; i < 100; ++i) {
+    dm *= 0.5;
+    const double xm = xa + dm;
+    const double fm = bis_f(bis_count_block(g, in, xm, scr), H, F);
+    if (fm * fa >= 0) xa = xm;
+    if (bis_stop(fm, dm, xm)) {
+      res = xm;
+      ok = true;
+      break;
+    }
+  }
+  if (threadIdx.x == 0) {
+    if (ok) {
+      s->offset = res;
+    } else {
+      s->status = CM_E_BISECT;
+      s->done = 1;
+      s->it += 1;  // the failed iteration counts
+    }
+    s->bis_state = 2;
+  }
+}
+
+// Whole bisection by one block (brackets that are not finite).
+template <typename T>
+__device__ void bis_whole_serial(const Geom& g, const Inst<T>& in, Scal* s, double* scr) {
+  const double xa = s->gmax, xb = s->gmin, H = s->H, F = s->F;
+  const double fa = bis_f(bis_count_block(g, in, xa, scr), H, F);
+  const double fb = bis_f(bis_count_block(g, in, xb, scr), H, F);
+  if (fa * fb > 0 || fa == 0 || fb == 0) {
+    if (threadIdx.x == 0) {
+      if (fa * fb > 0) {
+        s->status = CM_E_BISECT;
+        s->done = 1;
+        s->it += 1;
+      } else {
+        s->offset = fa == 0 ? xa : xb;
       }
+      s->bis_state = 2;
+    }
+    return;
+  }
+  bis_serial(g, in, s, xa, xb - xa, fa, 0, scr);
+}
+
+// One round of the batched bisection; launched kBisRounds times per
+// iteration on a grid (nblk, instances).  Round 0 starts from the brackets,
+// later rounds continue while the record's bis_state is 1.  The last round
+// finishes an unfinished bisection serially (more than kBisRounds *
+// kBisDepth steps: brackets about 2^48 xtol apart or more).
+template <typename T>
+__global__ void __launch_bounds__(256) k_bisect_round(Geom g, InstArr<T> ia, int round, int nblk) {
+  const Inst<T>& in = ia.get(blockIdx.y);
+  Scal* s = in.sc;
+  if (s->done || s->mode != MODE_MIX || !s->load_ctl) return;
+  if (round > 0 && s->bis_state != 1) return;
+  __shared__ double pts[512];            // the points, sorted, then NaN
+  __shared__ double raw[kBisPts];        // the points in tree order
+  __shared__ unsigned hist[kBisStride];  // gaps per bin: bin k = #points <= g
+  __shared__ unsigned long long cum[kBisStride];
+  __shared__ double scr[2 * 8 * 16];
+  __shared__ int s_last;
+  const double xa0 = round == 0 ? s->gmax : s->bis_xa;
+  const double dm0 = round == 0 ? s->gmin - s->gmax : s->bis_dm;
+  if (round == 0 && !(isfinite(s->gmax) && isfinite(s->gmin) && isfinite(dm0))) {
+    if (blockIdx.x == 0) bis_whole_serial(g, in, s, scr);
+    return;
+  }
+  const int np = round == 0 ? kBisPts : kBisTree;
+  for (int t = threadIdx.x; t < np; t += blockDim.x)
+    raw[t] = t < kBisTree ? bis_tree_point(xa0, dm0, t) : (t == kBisTree ? s->gmax : s->gmin);
+  for (int t = threadIdx.x; t < kBisStride; t += blockDim.x) hist[t] = 0u;
+  __syncthreads();
+  // sort by rank (ties in index order); the points are finite
+  for (int t = threadIdx.x; t < 512; t += blockDim.x) {
+    if (t < np) {
+      const double v = raw[t];
+      int r = 0;
+      for (int k = 0; k < np; ++k) {
+        const double u = raw[k];
+        r += (u < v || (u == v && k < t)) ? 1 : 0;
+      }
+      pts[r] = v;
+    } else {
+      pts[t] = __builtin_nan("");
+    }
+  }
+  __syncthreads();
+  const double lo = pts[0], hi = pts[np - 1];
+  // this block's share of the computational points, in row-major order
+  const size_t Mt = (size_t)g.nx * g.ny;
+  const size_t per = (Mt + nblk - 1) / nblk;
+  const size_t q0 = (size_t)blockIdx.x * per, q1 = q0 + per < Mt ? q0 + per : Mt;
+  const unsigned bd = blockDim.x;
+  const int di = (int)(bd / (unsigned)g.ny), dj = (int)(bd - (unsigned)di * (unsigned)g.ny);
+  size_t q = q0 + threadIdx.x;
+  int i = (int)(q / (size_t)g.ny), j = (int)(q - (size_t)i * g.ny);
+  unsigned below = 0, above = 0;
+  while (q < q1) {
+    double v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const bool ok = q < q1 && (!in.surf || in.surf[q]);
+      v[k] = ok ? in.w[(size_t)i * g.My + j] : __builtin_nan("");
+      q += bd;
       i += di;
       j += dj;
       if (j >= g.ny) {
@@ -1244,50 +1432,129 @@ __global__ void __launch_bounds__(1024) k_bisect(Geom g, InstArr<T> ia) {
         ++i;
       }
     }
-    int ops[1] = {RSUM};
-    double unused = 0.0;
-    block_reduce<1>(&v, &unused, ops, scr);
-    if (threadIdx.x == 0) s_cnt = v;
-    __syncthreads();
-    const double cnt = s_cnt;
-    __syncthreads();
-    return cnt * H - F;
-  };
-  double xa = s->gmax, xb = s->gmin;
-  const double fa = fval(xa), fb = fval(xb);
-  int status = 0;
-  double res = xa;
-  if (fa * fb > 0) {
-    status = CM_E_BISECT;
-  } else if (fa == 0) {
-    res = xa;
-  } else if (fb == 0) {
-    res = xb;
-  } else {
-    double dm = xb - xa;
-    bool ok = false;
-    for (int i = 0; i < 100; ++i) {
-      dm *= 0.5;
-      const double xm = xa + dm;
-      const double fm = fval(xm);
-      if (fm * fa >= 0) xa = xm;
-      if (fm == 0 || fabs(dm) < 2e-12 + 8.881784197001252e-16 * fabs(xm)) {
-        res = xm;
-        ok = true;
-        break;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double x = v[k];
+      if (x < lo) {
+        ++below;
+      } else if (x >= hi) {
+        ++above;
+      } else if (x == x) {
+        int pos = 0;
+#pragma unroll
+        for (int st = 256; st > 0; st >>= 1)
+          if (pts[pos + st - 1] <= x) pos += st;
+        atomicAdd(&hist[pos], 1u);
       }
     }
-    if (!ok) status = CM_E_BISECT;
   }
+  if (below) atomicAdd(&hist[0], below);
+  if (above) atomicAdd(&hist[np], above);
+  __syncthreads();
+  unsigned* part = (unsigned*)in.part;
+  unsigned* ctr = in.ctr;  // ticket slot 0 (set-up passes use it too, never at the same time)
+  for (int t = threadIdx.x; t <= np; t += blockDim.x)
+    __hip_atomic_store(&part[(size_t)blockIdx.x * kBisStride + t], hist[t], __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  wait_stores_done();
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = take_ticket(ctr, nblk, blockIdx.x);
+  __syncthreads();
+  if (!s_last) return;
+  acquire_agent();
+  // totals per bin, then the inclusive prefix: cum[k] = count(g < pts[k])
+  for (int t = threadIdx.x; t <= np; t += blockDim.x) {
+    unsigned long long c = 0;
+    for (int b = 0; b < nblk; ++b) c += part[(size_t)b * kBisStride + t];
+    cum[t] = c;
+  }
+  __syncthreads();
+  for (int off = 1; off <= np; off <<= 1) {
+    unsigned long long a0 = 0, a1 = 0;
+    const int t0 = threadIdx.x, t1 = threadIdx.x + blockDim.x;
+    if (t0 <= np && t0 >= off) a0 = cum[t0 - off];
+    if (t1 <= np && t1 >= off) a1 = cum[t1 - off];
+    __syncthreads();
+    if (t0 <= np) cum[t0] += a0;
+    if (t1 <= np) cum[t1] += a1;
+    __syncthreads();
+  }
+  // replay the steps (thread 0); the state goes back to the record
+  __shared__ double w_xa, w_dm, w_fa;
+  __shared__ int w_i, w_serial;
   if (threadIdx.x == 0) {
-    if (status) {
-      s->status = status;
+    __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // count(g < x) for x one of the points (-1: x is not one of them)
+    auto cnt_at = [&](double x) -> double {
+      int lb = 0;
+      for (int st = 256; st > 0; st >>= 1)
+        if (pts[lb + st - 1] < x) lb += st;
+      return (lb < np && pts[lb] == x) ? (double)cum[lb] : -1.0;
+    };
+    const double H = s->H, F = s->F;
+    double xa = xa0, dm = dm0, fa;
+    int it = 0, state = 1;  // 1 running, 2 finished
+    bool serial = false;
+    double res = 0.0;
+    if (round == 0) {
+      const double xb = s->gmin;
+      fa = bis_f(cnt_at(xa), H, F);
+      const double fb = bis_f(cnt_at(xb), H, F);
+      if (fa * fb > 0) {
+        state = 3;  // failed
+      } else if (fa == 0) {
+        res = xa;
+        state = 2;
+      } else if (fb == 0) {
+        res = xb;
+        state = 2;
+      }
+    } else {
+      fa = s->bis_fa;
+      it = s->bis_steps;
+    }
+    for (int lev = 0; state == 1 && lev < kBisDepth && it < 100; ++lev) {
+      const double dn = dm * 0.5;
+      const double xm = xa + dn;
+      const double c = cnt_at(xm);
+      if (c < 0.0) {  // not reached: the tree holds every point of these steps
+        serial = true;
+        break;
+      }
+      dm = dn;
+      const double fm = bis_f(c, H, F);
+      if (fm * fa >= 0) xa = xm;
+      ++it;
+      if (bis_stop(fm, dm, xm)) {
+        res = xm;
+        state = 2;
+      }
+    }
+    if (state == 1 && it >= 100) state = 3;
+    if (state == 1 && round + 1 >= kBisRounds) serial = true;
+    if (state == 2) {
+      s->offset = res;
+      s->bis_state = 2;
+    } else if (state == 3) {
+      s->status = CM_E_BISECT;
       s->done = 1;
       s->it += 1;  // the failed iteration counts
+      s->bis_state = 2;
     } else {
-      s->offset = res;
+      s->bis_xa = xa;
+      s->bis_dm = dm;
+      s->bis_fa = fa;
+      s->bis_steps = it;
+      s->bis_state = serial ? 2 : 1;
     }
+    w_xa = xa;
+    w_dm = dm;
+    w_fa = fa;
+    w_i = it;
+    w_serial = state == 1 && serial;
   }
+  __syncthreads();
+  if (w_serial) bis_serial(g, in, s, w_xa, w_dm, w_fa, w_i, scr);
 }
 
 // Conjugate direction t = c (g + beta t) on comp (:265-279) fused into the
@@ -2662,7 +2929,8 @@ WsLayout ws_layout(const cm_plan* P, int B) {
   L.st_t = al256(M * w);
   L.st_S = al256((M / 2) * 2 * w);
   L.st_sc = al256(sizeof(Scal) * (1 + kLogRing));  // the record and its log ring
-  L.st_part = al256(nparts * sizeof(double));
+  // (the bisection rounds keep their bin counts there too)
+  L.st_part = al256(std::max(nparts * sizeof(double), (size_t)bisect_blocks(P->ex.g) * kBisStride * sizeof(unsigned)));
   L.st_ctr = al256(8 * kCtrSlot * sizeof(unsigned));
   // FP32 plans: FP64 gap and forces, FP64 spectrum of the refresh transforms
   L.st_w = w == 4 ? al256(M * 8) : 0;
@@ -2745,6 +3013,15 @@ bool refresh_iteration(const cm_session* S, long long k) {
 // The second convolution of an iteration and the displacement pass: FP64 on
 // FP64 plans; FP32 plans take the FP64 transforms of the new forces on
 // refresh iterations and the FP32 transforms of the force change otherwise.
+// The rounds of the plastic step's bisection (every round after the one that
+// finishes it returns at once).
+template <typename T>
+void launch_bisect(const FftExec& ex, InstArr<T> ia, int B, hipStream_t st) {
+  const int nb = bisect_blocks(ex.g);
+  for (int r = 0; r < kBisRounds; ++r)
+    hipLaunchKernelGGL((k_bisect_round<T>), dim3(nb, B), dim3(256), 0, st, ex.g, ia, r, nb);
+}
+
 template <typename T, bool HARD>
 void enqueue_displacement_update(cm_session* S, InstArr<T> ia, bool refresh) {
   const int B = S->B;
@@ -2771,7 +3048,7 @@ void enqueue_iteration(cm_session* S, bool refresh) {
   hipStream_t st = S->st;
   // phase A of iteration k+1 is fused into ConsDisp of iteration k; the first
   // one runs at set-up
-  if (S->any_bisect) hipLaunchKernelGGL((k_bisect<T>), dim3(B), dim3(1024), 0, st, ex.g, ia);
+  if (S->any_bisect) launch_bisect<T>(ex, ia, B, st);
   launch_r2c<T, ProdDir<T>>(ex, ia, B, 3, st);
   launch_col<T, 3>(ex, ia, B, 0, 2, st);  // tau from the spectrum: no inverse pass
   if (S->any_hard) {
@@ -3016,7 +3293,7 @@ int session_profile(cm_session* S, int n, float* us) {
     HIPCHK(ctx, hipEventRecord(ev[k++], st));
     // (phase A is fused into the displacement pass: nothing to launch)
     HIPCHK(ctx, hipEventRecord(ev[k++], st));
-    if (S->any_bisect) hipLaunchKernelGGL((k_bisect<T>), dim3(B), dim3(1024), 0, st, ex.g, ia);
+    if (S->any_bisect) launch_bisect<T>(ex, ia, B, st);
     HIPCHK(ctx, hipEventRecord(ev[k++], st));
     launch_r2c<T, ProdDir<T>>(ex, ia, B, 3, st);
     HIPCHK(ctx, hipEventRecord(ev[k++], st));

@@ -44,6 +44,7 @@ def build(force=False):
         fh.write(_host_source())
     cmd = ["g++", "-std=c++17", "-O1", "-g", "-fPIC", "-shared", "-fsanitize=address", "-fno-omit-frame-pointer",
            "-I" + HERE, "-I" + os.path.join(ROOT, "contactmechanics_amd", "csrc"),
+           "-DCM_BIS_PER_BLOCK=160",  # small grids: the bisection rounds on several blocks
            "-o", LIB + ".tmp", gen, os.path.join(HERE, "emu_rt.cpp")]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:

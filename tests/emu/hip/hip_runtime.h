@@ -15,6 +15,8 @@
 #include <algorithm>
 using std::min;
 using std::max;
+using std::isfinite;
+inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz((unsigned)x); }
 
 #define __global__
 #define __device__

@@ -49,10 +49,14 @@ def solve(case):
     per = tuple(case.get("per", (0, 1)))
     tile = int(case.get("tile", 1))
     p, mx, my = plan(nx, ny * tile, bc, prec, sy=float(tile), periodicity=per, n_images=3, tile_y=tile)
-    h = heights(nx, ny, case["seed"])
-    kw = dict(external_force=case["force"], pentol=case["pentol"], forcetol=case["forcetol"], maxiter=600)
+    # hscale: the same problem with heights, load, hardness and pentol scaled
+    # (a wide bisection bracket)
+    hs = float(case.get("hscale", 1.0))
+    h = heights(nx, ny, case["seed"]) * hs
+    kw = dict(external_force=case["force"] * hs, pentol=case["pentol"] * hs, forcetol=case["forcetol"],
+              maxiter=int(case.get("maxiter", 600)))
     if case.get("hardness"):
-        kw["hardness"] = case["hardness"]
+        kw["hardness"] = case["hardness"] * hs
     dt = np.float64 if prec == 0 else np.float32
     u = np.zeros((mx, my), dt)
     f = np.zeros((mx, my), dt)
@@ -70,10 +74,12 @@ def solve(case):
         sub = SemiPeriodicHalfSpace((nx, ny), 1.0, (1.0, 1.0), periodicity=per, n_images=3)
     else:
         sub = (FreeHalfSpace if bc else PeriodicHalfSpace)((nx, ny), 1.0, (1.0, 1.0))
-    r = ccg(sub, h, **kw)
+    trace = []
+    r = ccg(sub, h, trace=trace, **kw)
     jac = -f[:nx, :ny].astype(np.float64)
     return dict(rc=rc, nit=st.nit, nit_oracle=int(r.nit), success_oracle=bool(r.success),
-                rel=float(np.max(np.abs(jac - r.jac)) / np.max(np.abs(r.jac))))
+                rel=float(np.max(np.abs(jac - r.jac)) / np.max(np.abs(r.jac))),
+                span=float(np.ptp(h)), mix=sum(t["status"] == "mix" for t in trace))
 
 
 def noalloc(case):

@@ -49,6 +49,20 @@ def test_emulated_plastic_free_solve_follows_oracle(emu_lib):
     assert r["rc"] == 0 and r["nit"] == r["nit_oracle"] and r["rel"] < 1e-8
 
 
+@pytest.mark.parametrize("hscale", [1.0, 1e5])
+def test_emulated_plastic_mixing_steps_follow_oracle(emu_lib, hscale):
+    # a hardness low enough for the plastic mixing step (10 of the first 40
+    # iterations), so the bisection runs; the emulated build counts the gaps
+    # in blocks of 160, so its rounds run on 5 blocks.  hscale 1e5: the same
+    # problem with a bracket about 2^51 xtol wide, past the 48 batched steps
+    # into the serial continuation
+    r = run_case(dict(kind="solve", nx=32, ny=24, bc=1, prec=0, seed=7, force=3e-3, pentol=1e-10,
+                      forcetol=1e-14, hardness=1e-5, hscale=hscale, maxiter=40))
+    assert r["mix"] == 10
+    assert r["nit"] == r["nit_oracle"] == 40
+    assert r["rel"] < 1e-10
+
+
 @pytest.mark.parametrize("bc,hardness", [(0, None), (1, 6e-5)])
 def test_emulated_fp32_solve_reaches_oracle_at_tight_tolerance(emu_lib, bc, hardness):
     # FP32 transforms with FP64 forces and gaps (incremental displacement
