@@ -1463,9 +1463,18 @@ __global__ void __launch_bounds__(256) k_bisect_round(Geom g, InstArr<T> ia, int
   if (!s_last) return;
   acquire_agent();
   // totals per bin, then the inclusive prefix: cum[k] = count(g < pts[k])
+  // (16 independent loads in flight per thread: one round trip per 16 blocks)
   for (int t = threadIdx.x; t <= np; t += blockDim.x) {
     unsigned long long c = 0;
-    for (int b = 0; b < nblk; ++b) c += part[(size_t)b * kBisStride + t];
+    int b = 0;
+    for (; b + 16 <= nblk; b += 16) {
+      unsigned v[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) v[k] = part[(size_t)(b + k) * kBisStride + t];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) c += v[k];
+    }
+    for (; b < nblk; ++b) c += part[(size_t)b * kBisStride + t];
     cum[t] = c;
   }
   __syncthreads();
