@@ -1235,28 +1235,39 @@ struct ElemPhaseA {
 // (a difference of distinct finite doubles is never rounded to zero; NaN
 // compares false either way).
 //
-// Batched rounds: the points that the next kBisDepth steps can visit form a
-// binary tree fixed by (xa, dm) alone (each step halves dm and moves xa to xm
-// or not), so one pass over the gaps counts the gaps below all 2^depth - 1 of
-// them (in round 0 also below the brackets), and the block that finishes the
-// pass replays the steps with those counts: the same points, counts and
-// decisions as the serial bisection, in ceil(steps / kBisDepth) passes over
-// the gaps by the whole grid instead of one pass per step by one block.
-constexpr int kBisDepth = 8;                          // steps per round
-constexpr int kBisTree = (1 << kBisDepth) - 1;        // points of a round's tree
-constexpr int kBisPts = kBisTree + 2;                 // round 0: and the brackets
-constexpr int kBisStride = 264;                       // bins (kBisPts + 1) per block partial, padded
-constexpr int kBisRounds = 6;                         // batched steps: kBisRounds * kBisDepth
+// Batched rounds: the points that the next k steps can visit form a binary
+// tree fixed by (xa, dm) alone (each step halves dm and moves xa to xm or
+// not), so one pass over the gaps counts the gaps below all 2^k - 1 of them
+// (in round 0 also below the brackets), and the block that finishes the pass
+// replays the steps with those counts: the same points, counts and decisions
+// as the serial bisection, in a few passes over the gaps by the whole grid
+// instead of one pass per step by one block.  Round 0 takes 8 steps (its
+// gaps spread over all bins: per-block bin counts, summed by the last block);
+// later rounds take 11 (the gaps left inside the bracket are few: their bins
+// go to a per-instance array with 64-bit atomics).
+constexpr int kBisDepth0 = 8;                         // steps of round 0
+constexpr int kBisDepth = 11;                         // steps of later rounds
+constexpr int kBisPts0 = (1 << kBisDepth0) + 1;       // round 0: tree and brackets
+constexpr int kBisSearch = 1 << kBisDepth;            // sorted points, NaN-padded
+constexpr int kBisStride = 264;                       // round-0 bins per block partial, padded
+constexpr int kBisRounds = 4;                         // batched steps: 8 + 3 * 11
 constexpr int kBisBlockMax = 128;
 #ifndef CM_BIS_PER_BLOCK
 #define CM_BIS_PER_BLOCK 8192  // the CPU emulation builds with fewer, so small grids span several blocks
 #endif
 constexpr int kBisPerBlock = CM_BIS_PER_BLOCK;        // gaps per block (32 per thread)
-static_assert(kBisPts + 1 <= kBisStride && kBisPts <= 512, "bisection tables");
+static_assert(kBisPts0 + 1 <= kBisStride && kBisPts0 < kBisSearch, "bisection tables");
 
 int bisect_blocks(const Geom& g) {
   const size_t Mt = (size_t)g.nx * g.ny;
   return (int)std::max<size_t>(1, std::min<size_t>(kBisBlockMax, (Mt + kBisPerBlock - 1) / kBisPerBlock));
+}
+
+// The per-instance bin counts of the rounds after the first, behind the
+// eight ticket slots (zero between rounds: the last block clears them).
+template <typename T>
+__device__ __forceinline__ unsigned long long* bis_bins(const Inst<T>& in) {
+  return (unsigned long long*)(in.ctr + 8 * kCtrSlot);
 }
 
 __device__ __forceinline__ double bis_f(double cnt, double H, double F) {
@@ -1268,8 +1279,8 @@ __device__ __forceinline__ bool bis_stop(double fm, double dm, double xm) {
   return fm == 0.0 || fabs(dm) < 2e-12 + 8.881784197001252e-16 * fabs(xm);
 }
 
-// Point t (level order) of the tree of the next kBisDepth steps from (xa, dm):
-// the bits of its path are the decisions "xa moved to xm".
+// Point t (level order) of the tree of the next steps from (xa, dm): the bits
+// of its path are the decisions "xa moved to xm".
 __device__ __forceinline__ double bis_tree_point(double xa, double dm, int t) {
   const int l = 31 - __clz(t + 1);
   const int p = t + 1 - (1 << l);
@@ -1281,6 +1292,186 @@ __device__ __forceinline__ double bis_tree_point(double xa, double dm, int t) {
   }
   d *= 0.5;
   return x + d;
+}
+
+// The np points of a round (the tree of `depth` steps, and with `brackets`
+// gmin and gmax) sorted ascending in pts[0, np), NaN in pts[np, kBisSearch).
+// In exact arithmetic in-order position along dm is the sorted order; rounding
+// can tie points (harmless) or, with dm near an ulp of xa, invert neighbours:
+// then they are sorted by rank.  Whole block; ends synchronised.
+__device__ void bis_place(double xa, double dm, int depth, bool brackets, double gmax, double gmin, double* pts,
+                          double* raw, int* flag) {
+  const int P = (1 << depth) - 1, np = P + (brackets ? 2 : 0), off = brackets ? 1 : 0;
+  if (threadIdx.x == 0) *flag = 0;
+  for (int t = threadIdx.x; t < kBisSearch; t += blockDim.x) {
+    if (t < np) {
+      double v;
+      int si;
+      if (t < P) {
+        v = bis_tree_point(xa, dm, t);
+        const int l = 31 - __clz(t + 1);
+        const int p = t + 1 - (1 << l);
+        const int io = ((2 * p + 1) << (depth - 1 - l)) - 1;
+        si = (dm < 0.0 ? P - 1 - io : io) + off;
+      } else {
+        v = t == P ? gmax : gmin;
+        si = t == P ? np - 1 : 0;
+      }
+      raw[t] = v;
+      pts[si] = v;
+    } else {
+      pts[t] = __builtin_nan("");
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t + 1 < np; t += blockDim.x)
+    if (!(pts[t] <= pts[t + 1])) *flag = 1;
+  __syncthreads();
+  if (*flag) {
+    for (int t = threadIdx.x; t < np; t += blockDim.x) {
+      const double v = raw[t];
+      int r = 0;
+      for (int k = 0; k < np; ++k) {
+        const double u = raw[k];
+        r += (u < v || (u == v && k < t)) ? 1 : 0;
+      }
+      pts[r] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// Bins the gaps of the computational points [q0, q1) (row-major) into hist:
+// bin k = number of points <= g; NaN gaps and masked points are not counted.
+// Whole block; ends synchronised.
+template <typename T>
+__device__ void bis_bin_gaps(const Geom& g, const Inst<T>& in, size_t q0, size_t q1, const double* pts, int np,
+                             unsigned* hist) {
+  const double lo = pts[0], hi = pts[np - 1];
+  const unsigned bd = blockDim.x;
+  const int di = (int)(bd / (unsigned)g.ny), dj = (int)(bd - (unsigned)di * (unsigned)g.ny);
+  size_t q = q0 + threadIdx.x;
+  int i = (int)(q / (size_t)g.ny), j = (int)(q - (size_t)i * g.ny);
+  unsigned below = 0, above = 0;
+  while (q < q1) {
+    double v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const bool ok = q < q1 && (!in.surf || in.surf[q]);
+      v[k] = ok ? in.w[(size_t)i * g.My + j] : __builtin_nan("");
+      q += bd;
+      i += di;
+      j += dj;
+      if (j >= g.ny) {
+        j -= g.ny;
+        ++i;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double x = v[k];
+      if (x < lo) {
+        ++below;
+      } else if (x >= hi) {
+        ++above;
+      } else if (x == x) {
+        int pos = 0;
+#pragma unroll
+        for (int st = kBisSearch / 2; st > 0; st >>= 1)
+          if (pts[pos + st - 1] <= x) pos += st;
+        atomicAdd(&hist[pos], 1u);
+      }
+    }
+  }
+  if (below) atomicAdd(&hist[0], below);
+  if (above) atomicAdd(&hist[np], above);
+  __syncthreads();
+}
+
+// Inclusive prefix of cum[0, n) in place.  Whole block; ends synchronised.
+__device__ void bis_scan(unsigned long long* cum, int n) {
+  constexpr int kPer = kBisSearch / 256;
+  for (int off = 1; off < n; off <<= 1) {
+    unsigned long long a[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int t = threadIdx.x + k * (int)blockDim.x;
+      a[k] = (t < n && t >= off) ? cum[t - off] : 0ull;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int t = threadIdx.x + k * (int)blockDim.x;
+      if (t < n) cum[t] += a[k];
+    }
+    __syncthreads();
+  }
+}
+
+// Bisection state between rounds and passes.
+struct BisState {
+  double xa, dm, fa, res;
+  int it, state;  // 1 running, 2 found res, 3 failed
+};
+
+// Replays up to `depth` steps with the prefix counts cum of the sorted points
+// (thread 0).  Round 0 (`first`) starts with the brackets' checks.
+__device__ void bis_walk(BisState& b, bool first, int depth, const double* pts, int np,
+                         const unsigned long long* cum, double H, double F, double gmax, double gmin) {
+  // count(g < x) for x one of the points (-1: x is not one of them)
+  auto cnt_at = [&](double x) -> double {
+    int lb = 0;
+    for (int st = kBisSearch / 2; st > 0; st >>= 1)
+      if (pts[lb + st - 1] < x) lb += st;
+    return (lb < np && pts[lb] == x) ? (double)cum[lb] : -1.0;
+  };
+  if (first) {
+    b.xa = gmax;
+    b.dm = gmin - gmax;
+    b.it = 0;
+    b.state = 1;
+    b.fa = bis_f(cnt_at(gmax), H, F);
+    const double fb = bis_f(cnt_at(gmin), H, F);
+    if (b.fa * fb > 0) {
+      b.state = 3;
+    } else if (b.fa == 0) {
+      b.res = gmax;
+      b.state = 2;
+    } else if (fb == 0) {
+      b.res = gmin;
+      b.state = 2;
+    }
+  }
+  for (int lev = 0; b.state == 1 && lev < depth && b.it < 100; ++lev) {
+    const double dn = b.dm * 0.5;
+    const double xm = b.xa + dn;
+    const double c = cnt_at(xm);
+    if (c < 0.0) break;  // not reached (the tree holds every point of these steps): no step taken
+    b.dm = dn;
+    const double fm = bis_f(c, H, F);
+    if (fm * b.fa >= 0) b.xa = xm;
+    ++b.it;
+    if (bis_stop(fm, b.dm, xm)) {
+      b.res = xm;
+      b.state = 2;
+    }
+  }
+  if (b.state == 1 && b.it >= 100) b.state = 3;
+}
+
+__device__ __forceinline__ void bis_commit(Scal* s, const BisState& b) {
+  if (b.state == 2) {
+    s->offset = b.res;
+  } else if (b.state == 3) {
+    s->status = CM_E_BISECT;
+    s->done = 1;
+    s->it += 1;  // the failed iteration counts
+  }
+  s->bis_xa = b.xa;
+  s->bis_dm = b.dm;
+  s->bis_fa = b.fa;
+  s->bis_steps = b.it;
+  s->bis_state = b.state == 1 ? 1 : 2;
 }
 
 // count(g < x) over the computational points by one whole block (every
@@ -1312,258 +1503,171 @@ __device__ double bis_count_block(const Geom& g, const Inst<T>& in, double x, do
   return cnt;
 }
 
-// Serial bisection steps i.. by one whole block from (xa, dm, fa), one count
-// per step; writes the offset or the failure.
-template <typename T>
-__device__ void bis_serial(const Geom& g, const Inst<T>& in, Scal* s, double xa, double dm, double fa, int i,
-                           double* scr) {
-  const double H = s->H, F = s->F;
-  double res = 0.0;
-  bool ok = false;
-  for (; i < 100; ++i) {
-    dm *= 0.5;
-    const double xm = xa + dm;
-    const double fm = bis_f(bis_count_block(g, in, xm, scr), H, F);
-    if (fm * fa >= 0) xa = xm;
-    if (bis_stop(fm, dm, xm)) {
-      res = xm;
-      ok = true;
-      break;
-    }
-  }
-  if (threadIdx.x == 0) {
-    if (ok) {
-      s->offset = res;
-    } else {
-      s->status = CM_E_BISECT;
-      s->done = 1;
-      s->it += 1;  // the failed iteration counts
-    }
-    s->bis_state = 2;
-  }
-}
-
-// Whole bisection by one block (brackets that are not finite).
+// Whole bisection by one block, one count per step (brackets that are not
+// finite: scipy's checks and steps on whatever the counts give).
 template <typename T>
 __device__ void bis_whole_serial(const Geom& g, const Inst<T>& in, Scal* s, double* scr) {
-  const double xa = s->gmax, xb = s->gmin, H = s->H, F = s->F;
-  const double fa = bis_f(bis_count_block(g, in, xa, scr), H, F);
+  const double H = s->H, F = s->F;
+  BisState b{};
+  b.xa = s->gmax;
+  const double xb = s->gmin;
+  b.fa = bis_f(bis_count_block(g, in, b.xa, scr), H, F);
   const double fb = bis_f(bis_count_block(g, in, xb, scr), H, F);
-  if (fa * fb > 0 || fa == 0 || fb == 0) {
-    if (threadIdx.x == 0) {
-      if (fa * fb > 0) {
-        s->status = CM_E_BISECT;
-        s->done = 1;
-        s->it += 1;
-      } else {
-        s->offset = fa == 0 ? xa : xb;
-      }
-      s->bis_state = 2;
-    }
-    return;
+  b.state = 1;
+  if (b.fa * fb > 0) {
+    b.state = 3;
+  } else if (b.fa == 0) {
+    b.res = b.xa;
+    b.state = 2;
+  } else if (fb == 0) {
+    b.res = xb;
+    b.state = 2;
   }
-  bis_serial(g, in, s, xa, xb - xa, fa, 0, scr);
+  b.dm = xb - b.xa;
+  for (; b.state == 1 && b.it < 100;) {
+    b.dm *= 0.5;
+    const double xm = b.xa + b.dm;
+    const double fm = bis_f(bis_count_block(g, in, xm, scr), H, F);
+    if (fm * b.fa >= 0) b.xa = xm;
+    ++b.it;
+    if (bis_stop(fm, b.dm, xm)) {
+      b.res = xm;
+      b.state = 2;
+    }
+  }
+  if (b.state == 1) b.state = 3;
+  if (threadIdx.x == 0) bis_commit(s, b);
 }
 
 // One round of the batched bisection; launched kBisRounds times per
 // iteration on a grid (nblk, instances).  Round 0 starts from the brackets,
-// later rounds continue while the record's bis_state is 1.  The last round
-// finishes an unfinished bisection serially (more than kBisRounds *
-// kBisDepth steps: brackets about 2^48 xtol apart or more).
+// later rounds continue while the record's bis_state is 1.  The last round's
+// last block finishes an unfinished bisection alone, kBisDepth steps per pass
+// over the gaps (more than 8 + 3 * 11 steps: brackets about 2^41 xtol apart
+// or more).
 template <typename T>
 __global__ void __launch_bounds__(256) k_bisect_round(Geom g, InstArr<T> ia, int round, int nblk) {
   const Inst<T>& in = ia.get(blockIdx.y);
   Scal* s = in.sc;
   if (s->done || s->mode != MODE_MIX || !s->load_ctl) return;
   if (round > 0 && s->bis_state != 1) return;
-  __shared__ double pts[512];            // the points, sorted, then NaN
-  __shared__ double raw[kBisPts];        // the points in tree order
-  __shared__ unsigned hist[kBisStride];  // gaps per bin: bin k = #points <= g
-  __shared__ unsigned long long cum[kBisStride];
+  __shared__ double pts[kBisSearch];
+  __shared__ union {
+    double raw[kBisSearch];               // the points in tree order
+    unsigned long long cum[kBisSearch];  // then the prefix counts
+  } ub;
+  __shared__ unsigned hist[kBisSearch];  // gaps per bin: bin k = #points <= g
   __shared__ double scr[2 * 8 * 16];
-  __shared__ int s_last;
-  const double xa0 = round == 0 ? s->gmax : s->bis_xa;
-  const double dm0 = round == 0 ? s->gmin - s->gmax : s->bis_dm;
-  if (round == 0 && !(isfinite(s->gmax) && isfinite(s->gmin) && isfinite(dm0))) {
+  __shared__ int s_last, s_flag;
+  __shared__ BisState s_b;
+  const double gmax = s->gmax, gmin = s->gmin, H = s->H, F = s->F;
+  const bool first = round == 0;
+  const double xa0 = first ? gmax : s->bis_xa;
+  const double dm0 = first ? gmin - gmax : s->bis_dm;
+  if (first && !(isfinite(gmax) && isfinite(gmin) && isfinite(dm0))) {
     if (blockIdx.x == 0) bis_whole_serial(g, in, s, scr);
     return;
   }
-  const int np = round == 0 ? kBisPts : kBisTree;
-  for (int t = threadIdx.x; t < np; t += blockDim.x)
-    raw[t] = t < kBisTree ? bis_tree_point(xa0, dm0, t) : (t == kBisTree ? s->gmax : s->gmin);
-  for (int t = threadIdx.x; t < kBisStride; t += blockDim.x) hist[t] = 0u;
-  __syncthreads();
-  // sort by rank (ties in index order); the points are finite
-  for (int t = threadIdx.x; t < 512; t += blockDim.x) {
-    if (t < np) {
-      const double v = raw[t];
-      int r = 0;
-      for (int k = 0; k < np; ++k) {
-        const double u = raw[k];
-        r += (u < v || (u == v && k < t)) ? 1 : 0;
-      }
-      pts[r] = v;
-    } else {
-      pts[t] = __builtin_nan("");
-    }
-  }
-  __syncthreads();
-  const double lo = pts[0], hi = pts[np - 1];
+  const int depth = first ? kBisDepth0 : kBisDepth;
+  const int np = (1 << depth) - 1 + (first ? 2 : 0);
+  for (int t = threadIdx.x; t < kBisSearch; t += blockDim.x) hist[t] = 0u;
+  bis_place(xa0, dm0, depth, first, gmax, gmin, pts, ub.raw, &s_flag);
   // this block's share of the computational points, in row-major order
   const size_t Mt = (size_t)g.nx * g.ny;
   const size_t per = (Mt + nblk - 1) / nblk;
   const size_t q0 = (size_t)blockIdx.x * per, q1 = q0 + per < Mt ? q0 + per : Mt;
-  const unsigned bd = blockDim.x;
-  const int di = (int)(bd / (unsigned)g.ny), dj = (int)(bd - (unsigned)di * (unsigned)g.ny);
-  size_t q = q0 + threadIdx.x;
-  int i = (int)(q / (size_t)g.ny), j = (int)(q - (size_t)i * g.ny);
-  unsigned below = 0, above = 0;
-  while (q < q1) {
-    double v[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const bool ok = q < q1 && (!in.surf || in.surf[q]);
-      v[k] = ok ? in.w[(size_t)i * g.My + j] : __builtin_nan("");
-      q += bd;
-      i += di;
-      j += dj;
-      if (j >= g.ny) {
-        j -= g.ny;
-        ++i;
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const double x = v[k];
-      if (x < lo) {
-        ++below;
-      } else if (x >= hi) {
-        ++above;
-      } else if (x == x) {
-        int pos = 0;
-#pragma unroll
-        for (int st = 256; st > 0; st >>= 1)
-          if (pts[pos + st - 1] <= x) pos += st;
-        atomicAdd(&hist[pos], 1u);
-      }
-    }
-  }
-  if (below) atomicAdd(&hist[0], below);
-  if (above) atomicAdd(&hist[np], above);
-  __syncthreads();
+  bis_bin_gaps(g, in, q0 < Mt ? q0 : Mt, q1, pts, np, hist);
   unsigned* part = (unsigned*)in.part;
+  unsigned long long* gb = bis_bins(in);
   unsigned* ctr = in.ctr;  // ticket slot 0 (set-up passes use it too, never at the same time)
-  for (int t = threadIdx.x; t <= np; t += blockDim.x)
-    __hip_atomic_store(&part[(size_t)blockIdx.x * kBisStride + t], hist[t], __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+  if (first) {
+    for (int t = threadIdx.x; t <= np; t += blockDim.x)
+      __hip_atomic_store(&part[(size_t)blockIdx.x * kBisStride + t], hist[t], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    for (int t = threadIdx.x; t <= np; t += blockDim.x)
+      if (hist[t]) atomicAdd(&gb[t], (unsigned long long)hist[t]);
+  }
   wait_stores_done();
   __syncthreads();
   if (threadIdx.x == 0) s_last = take_ticket(ctr, nblk, blockIdx.x);
   __syncthreads();
   if (!s_last) return;
   acquire_agent();
-  // totals per bin, then the inclusive prefix: cum[k] = count(g < pts[k])
-  // (16 independent loads in flight per thread: one round trip per 16 blocks)
-  for (int t = threadIdx.x; t <= np; t += blockDim.x) {
-    unsigned long long c = 0;
-    int b = 0;
-    for (; b + 16 <= nblk; b += 16) {
-      unsigned v[16];
+  if (first) {
+    // (16 independent loads in flight per thread: one round trip per 16 blocks)
+    for (int t = threadIdx.x; t <= np; t += blockDim.x) {
+      unsigned long long c = 0;
+      int b = 0;
+      for (; b + 16 <= nblk; b += 16) {
+        unsigned v[16];
 #pragma unroll
-      for (int k = 0; k < 16; ++k) v[k] = part[(size_t)(b + k) * kBisStride + t];
+        for (int k = 0; k < 16; ++k) v[k] = part[(size_t)(b + k) * kBisStride + t];
 #pragma unroll
-      for (int k = 0; k < 16; ++k) c += v[k];
+        for (int k = 0; k < 16; ++k) c += v[k];
+      }
+      for (; b < nblk; ++b) c += part[(size_t)b * kBisStride + t];
+      ub.cum[t] = c;
     }
-    for (; b < nblk; ++b) c += part[(size_t)b * kBisStride + t];
-    cum[t] = c;
+  } else {
+    for (int t = threadIdx.x; t <= np; t += blockDim.x) {
+      ub.cum[t] = gb[t];
+      __hip_atomic_store(&gb[t], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
   __syncthreads();
-  for (int off = 1; off <= np; off <<= 1) {
-    unsigned long long a0 = 0, a1 = 0;
-    const int t0 = threadIdx.x, t1 = threadIdx.x + blockDim.x;
-    if (t0 <= np && t0 >= off) a0 = cum[t0 - off];
-    if (t1 <= np && t1 >= off) a1 = cum[t1 - off];
-    __syncthreads();
-    if (t0 <= np) cum[t0] += a0;
-    if (t1 <= np) cum[t1] += a1;
-    __syncthreads();
-  }
-  // replay the steps (thread 0); the state goes back to the record
-  __shared__ double w_xa, w_dm, w_fa;
-  __shared__ int w_i, w_serial;
+  bis_scan(ub.cum, np + 1);
   if (threadIdx.x == 0) {
     __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    // count(g < x) for x one of the points (-1: x is not one of them)
-    auto cnt_at = [&](double x) -> double {
-      int lb = 0;
-      for (int st = 256; st > 0; st >>= 1)
-        if (pts[lb + st - 1] < x) lb += st;
-      return (lb < np && pts[lb] == x) ? (double)cum[lb] : -1.0;
-    };
-    const double H = s->H, F = s->F;
-    double xa = xa0, dm = dm0, fa;
-    int it = 0, state = 1;  // 1 running, 2 finished
-    bool serial = false;
-    double res = 0.0;
-    if (round == 0) {
-      const double xb = s->gmin;
-      fa = bis_f(cnt_at(xa), H, F);
-      const double fb = bis_f(cnt_at(xb), H, F);
-      if (fa * fb > 0) {
-        state = 3;  // failed
-      } else if (fa == 0) {
-        res = xa;
-        state = 2;
-      } else if (fb == 0) {
-        res = xb;
-        state = 2;
-      }
-    } else {
-      fa = s->bis_fa;
-      it = s->bis_steps;
+    BisState b{};
+    if (!first) {
+      b.xa = xa0;
+      b.dm = dm0;
+      b.fa = s->bis_fa;
+      b.it = s->bis_steps;
+      b.state = 1;
     }
-    for (int lev = 0; state == 1 && lev < kBisDepth && it < 100; ++lev) {
-      const double dn = dm * 0.5;
-      const double xm = xa + dn;
-      const double c = cnt_at(xm);
-      if (c < 0.0) {  // not reached: the tree holds every point of these steps
-        serial = true;
-        break;
-      }
-      dm = dn;
-      const double fm = bis_f(c, H, F);
-      if (fm * fa >= 0) xa = xm;
-      ++it;
-      if (bis_stop(fm, dm, xm)) {
-        res = xm;
-        state = 2;
-      }
-    }
-    if (state == 1 && it >= 100) state = 3;
-    if (state == 1 && round + 1 >= kBisRounds) serial = true;
-    if (state == 2) {
-      s->offset = res;
-      s->bis_state = 2;
-    } else if (state == 3) {
-      s->status = CM_E_BISECT;
-      s->done = 1;
-      s->it += 1;  // the failed iteration counts
-      s->bis_state = 2;
-    } else {
-      s->bis_xa = xa;
-      s->bis_dm = dm;
-      s->bis_fa = fa;
-      s->bis_steps = it;
-      s->bis_state = serial ? 2 : 1;
-    }
-    w_xa = xa;
-    w_dm = dm;
-    w_fa = fa;
-    w_i = it;
-    w_serial = state == 1 && serial;
+    bis_walk(b, first, depth, pts, np, ub.cum, H, F, gmax, gmin);
+    s_b = b;
   }
   __syncthreads();
-  if (w_serial) bis_serial(g, in, s, w_xa, w_dm, w_fa, w_i, scr);
+  // the last round: this block goes on alone, a tree of kBisDepth steps per
+  // pass over all the gaps
+  while (s_b.state == 1 && round + 1 >= kBisRounds) {
+    const BisState b0 = s_b;
+    for (int t = threadIdx.x; t < kBisSearch; t += blockDim.x) hist[t] = 0u;
+    bis_place(b0.xa, b0.dm, kBisDepth, false, gmax, gmin, pts, ub.raw, &s_flag);
+    const int n1 = (1 << kBisDepth) - 1;
+    bis_bin_gaps(g, in, 0, Mt, pts, n1, hist);
+    for (int t = threadIdx.x; t <= n1; t += blockDim.x) ub.cum[t] = hist[t];
+    __syncthreads();
+    bis_scan(ub.cum, n1 + 1);
+    if (threadIdx.x == 0) {
+      BisState b = b0;
+      bis_walk(b, false, kBisDepth, pts, n1, ub.cum, H, F, gmax, gmin);
+      s_b = b;
+    }
+    __syncthreads();
+    if (s_b.state == 1 && s_b.it == b0.it) {
+      // no step taken (not reached): one step with its own count, so every
+      // pass of this loop takes at least one of the at most 100 steps
+      BisState b = b0;
+      b.dm *= 0.5;
+      const double xm = b.xa + b.dm;
+      const double fm = bis_f(bis_count_block(g, in, xm, scr), H, F);
+      if (fm * b.fa >= 0) b.xa = xm;
+      ++b.it;
+      if (bis_stop(fm, b.dm, xm)) {
+        b.res = xm;
+        b.state = 2;
+      } else if (b.it >= 100) {
+        b.state = 3;
+      }
+      if (threadIdx.x == 0) s_b = b;
+      __syncthreads();
+    }
+  }
+  if (threadIdx.x == 0) bis_commit(s, s_b);
 }
 
 // Conjugate direction t = c (g + beta t) on comp (:265-279) fused into the
@@ -2940,7 +3044,8 @@ WsLayout ws_layout(const cm_plan* P, int B) {
   L.st_sc = al256(sizeof(Scal) * (1 + kLogRing));  // the record and its log ring
   // (the bisection rounds keep their bin counts there too)
   L.st_part = al256(std::max(nparts * sizeof(double), (size_t)bisect_blocks(P->ex.g) * kBisStride * sizeof(unsigned)));
-  L.st_ctr = al256(8 * kCtrSlot * sizeof(unsigned));
+  // the tickets, then the bisection's bin counts
+  L.st_ctr = al256(8 * kCtrSlot * sizeof(unsigned) + kBisSearch * sizeof(unsigned long long));
   // FP32 plans: FP64 gap and forces, FP64 spectrum of the refresh transforms
   L.st_w = w == 4 ? al256(M * 8) : 0;
   L.st_fm = w == 4 ? al256(M * 8) : 0;

@@ -167,6 +167,11 @@ inline unsigned atomicAdd(unsigned* p, unsigned v) {
   *p = o + v;
   return o;
 }
+inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) {
+  unsigned long long o = *p;
+  *p = o + v;
+  return o;
+}
 #define __ATOMIC_RELAXED_EMU 0
 #define __HIP_MEMORY_SCOPE_AGENT 0
 template <typename A, typename B>
