@@ -216,7 +216,8 @@ def bench_sweep(args):
     M = (2 * n) ** 2
     line = {
         "metric": "C3 plastic load sweep, grid-pt-iters/s (FP64, warm-started)",
-        "value": M * nit / seconds, "unit": "grid-pt-iters/s", "n_gpus": 1, "higher_is_better": True,
+        "value": M * nit / seconds, "unit": "grid-pt-iters/s", "n_gpus": 1, "steps": nit, "warmup": 3,
+        "ms_per_step": 1e3 * seconds / nit, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": cfg["name"]},
         "seconds": seconds, "iterations": nit, "loads": loads,
