@@ -1540,17 +1540,22 @@ __device__ void bis_whole_serial(const Geom& g, const Inst<T>& in, Scal* s, doub
 }
 
 // One round of the batched bisection; launched kBisRounds times per
-// iteration on a grid (nblk, instances).  Round 0 starts from the brackets,
+// iteration on a grid (nblk, instances); scb + b * sc_stride is instance b's
+// scalar record.  Round 0 starts from the brackets,
 // later rounds continue while the record's bis_state is 1.  The last round's
 // last block finishes an unfinished bisection alone, kBisDepth steps per pass
 // over the gaps (more than 8 + 3 * 11 steps: brackets about 2^41 xtol apart
 // or more).
 template <typename T>
-__global__ void __launch_bounds__(256) k_bisect_round(Geom g, InstArr<T> ia, int round, int nblk) {
-  const Inst<T>& in = ia.get(blockIdx.y);
-  Scal* s = in.sc;
+__global__ void __launch_bounds__(256) k_bisect_round(Geom g, InstArr<T> ia, char* scb, size_t sc_stride, int round,
+                                                      int nblk) {
+  // the scalar record first, addressed from the arguments (instance b's at
+  // scb + b * sc_stride, as in the instance record): a round with nothing to
+  // do makes one dependent load, not two
+  Scal* s = (Scal*)(scb + (size_t)blockIdx.y * sc_stride);
   if (s->done || s->mode != MODE_MIX || !s->load_ctl) return;
   if (round > 0 && s->bis_state != 1) return;
+  const Inst<T>& in = ia.get(blockIdx.y);
   __shared__ double pts[kBisSearch];
   __shared__ union {
     double raw[kBisSearch];               // the points in tree order
@@ -3130,10 +3135,11 @@ bool refresh_iteration(const cm_session* S, long long k) {
 // The rounds of the plastic step's bisection (every round after the one that
 // finishes it returns at once).
 template <typename T>
-void launch_bisect(const FftExec& ex, InstArr<T> ia, int B, hipStream_t st) {
+void launch_bisect(const cm_session* S, const FftExec& ex, InstArr<T> ia, int B, hipStream_t st) {
   const int nb = bisect_blocks(ex.g);
+  char* scb = (char*)S->ws + S->L.sc;
   for (int r = 0; r < kBisRounds; ++r)
-    hipLaunchKernelGGL((k_bisect_round<T>), dim3(nb, B), dim3(256), 0, st, ex.g, ia, r, nb);
+    hipLaunchKernelGGL((k_bisect_round<T>), dim3(nb, B), dim3(256), 0, st, ex.g, ia, scb, S->L.st_sc, r, nb);
 }
 
 template <typename T, bool HARD>
@@ -3162,7 +3168,7 @@ void enqueue_iteration(cm_session* S, bool refresh) {
   hipStream_t st = S->st;
   // phase A of iteration k+1 is fused into ConsDisp of iteration k; the first
   // one runs at set-up
-  if (S->any_bisect) launch_bisect<T>(ex, ia, B, st);
+  if (S->any_bisect) launch_bisect<T>(S, ex, ia, B, st);
   launch_r2c<T, ProdDir<T>>(ex, ia, B, 3, st);
   launch_col<T, 3>(ex, ia, B, 0, 2, st);  // tau from the spectrum: no inverse pass
   if (S->any_hard) {
@@ -3407,7 +3413,7 @@ int session_profile(cm_session* S, int n, float* us) {
     HIPCHK(ctx, hipEventRecord(ev[k++], st));
     // (phase A is fused into the displacement pass: nothing to launch)
     HIPCHK(ctx, hipEventRecord(ev[k++], st));
-    if (S->any_bisect) launch_bisect<T>(ex, ia, B, st);
+    if (S->any_bisect) launch_bisect<T>(S, ex, ia, B, st);
     HIPCHK(ctx, hipEventRecord(ev[k++], st));
     launch_r2c<T, ProdDir<T>>(ex, ia, B, 3, st);
     HIPCHK(ctx, hipEventRecord(ev[k++], st));
